@@ -1,0 +1,425 @@
+// Per-cell and per-interface numerics of the relaxation-GRP MUSCL scheme.
+//
+// Everything works on a "line state" Q = (rho, p, un, ut): the density, the
+// pressure, the velocity normal to the interfaces of the sweep and the
+// tangential velocity.  An X-sweep uses (un, ut) = (u, v); a Y-sweep uses
+// (un, ut) = (v, u), which is exactly the u<->v swap the reference applies
+// with rotate_and_flip_vertically (src/basicFuns.cuh:249-262) before running
+// its X kernels on the rotated array.
+//
+// Two instantiations:
+//   Exact = true  : the reference's expressions with its evaluation order
+//                   (src/basicFuns.cuh:14-212); compiled without FMA
+//                   contraction it reproduces the host FP64 restatement bit
+//                   for bit.
+//   Exact = false : the same mathematics with reciprocals shared between
+//                   terms (8 reciprocals + 2 square roots per interface
+//                   instead of ~62 divisions + 4 square roots) and FMAs.
+//                   Results differ from Exact only by rounding.
+#pragma once
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define FVM_HD __host__ __device__ __forceinline__
+#else
+#define FVM_HD inline
+#endif
+
+namespace fvm2d {
+
+struct Q {
+    double r, p, un, ut;
+};
+
+// Interface record of the GRP: state (rho, p, un, ut, e) and its time
+// derivative; e is the specific internal energy.
+struct IfState {
+    double r, p, un, ut, e;
+};
+
+struct Gas {
+    double gamma;  // adiabatic exponent
+    double g1;     // gamma - 1
+    double g1i;    // 1 / (gamma - 1)
+    double kappa;  // (gamma + 1) / 2
+};
+
+FVM_HD Gas make_gas(double gamma) {
+    Gas g;
+    g.gamma = gamma;
+    g.g1 = gamma - 1.0;
+    g.g1i = 1.0 / (gamma - 1.0);
+    g.kappa = (gamma + 1.0) / 2.0;
+    return g;
+}
+
+// Geometry of one sweep direction: h is the cell size along the sweep.
+struct Line {
+    double h;
+    double ih;     // 1.0 / h
+    double ih_a;   // (1.0 / h) * alpha
+    double ih_c;   // 0.5 * (1.0 / h)
+    double hh;     // 0.5 * h
+    double alpha;
+};
+
+FVM_HD Line make_line(double h, double alpha) {
+    Line L;
+    L.h = h;
+    L.ih = 1.0 / h;
+    L.ih_a = L.ih * alpha;
+    L.ih_c = 0.5 * L.ih;
+    L.hh = 0.5 * h;
+    L.alpha = alpha;
+    return L;
+}
+
+FVM_HD double tmin(double a, double b) { return b < a ? b : a; }
+FVM_HD double tmax(double a, double b) { return a < b ? b : a; }
+
+// ---------------------------------------------------------------- limiter
+// src/basicFuns.cuh:49-68: generalised minmod of the central difference
+// (weight 1/2) and the one-sided differences (weight alpha).
+template <bool Exact>
+FVM_HD double limit1(double qL, double q, double qR, const Line &L) {
+    double c, l, r;
+    if (Exact) {
+        c = (qR - qL) * 0.5 * L.ih;
+        l = (q - qL) * L.ih * L.alpha;
+        r = (qR - q) * L.ih * L.alpha;
+    } else {
+        c = (qR - qL) * L.ih_c;
+        l = (q - qL) * L.ih_a;
+        r = (qR - q) * L.ih_a;
+    }
+    double s13 = c * r, s23 = l * r;
+    if (s13 > 0.0 && s23 > 0.0) {
+        if (Exact) {
+            double sgn = 1.0;
+            if (c <= 0.0) sgn = -1.0;
+            return sgn * tmin(fabs(c), tmin(fabs(l), fabs(r)));
+        } else {
+            // all three share the sign of c here
+            return copysign(fmin(fabs(c), fmin(fabs(l), fabs(r))), c);
+        }
+    }
+    return 0.0;
+}
+
+template <bool Exact>
+FVM_HD Q slope(const Q &qL, const Q &q, const Q &qR, const Line &L) {
+    return Q{limit1<Exact>(qL.r, q.r, qR.r, L), limit1<Exact>(qL.p, q.p, qR.p, L),
+             limit1<Exact>(qL.un, q.un, qR.un, L), limit1<Exact>(qL.ut, q.ut, qR.ut, L)};
+}
+
+// src/basicFuns.cuh:69-70: face states q -/+ sigma*h/2.
+template <bool Exact>
+FVM_HD Q face_minus(const Q &q, const Q &s, const Line &L) {
+    if (Exact)
+        return Q{q.r - s.r * L.h * 0.5, q.p - s.p * L.h * 0.5, q.un - s.un * L.h * 0.5,
+                 q.ut - s.ut * L.h * 0.5};
+    return Q{fma(-s.r, L.hh, q.r), fma(-s.p, L.hh, q.p), fma(-s.un, L.hh, q.un),
+             fma(-s.ut, L.hh, q.ut)};
+}
+
+template <bool Exact>
+FVM_HD Q face_plus(const Q &q, const Q &s, const Line &L) {
+    if (Exact)
+        return Q{q.r + s.r * L.h * 0.5, q.p + s.p * L.h * 0.5, q.un + s.un * L.h * 0.5,
+                 q.ut + s.ut * L.h * 0.5};
+    return Q{fma(s.r, L.hh, q.r), fma(s.p, L.hh, q.p), fma(s.un, L.hh, q.un),
+             fma(s.ut, L.hh, q.ut)};
+}
+
+// ---------------------------------------------------------------- GRP
+// Relaxation GRP at one interface, src/basicFuns.cuh:86-189.
+//   fl, fr : left/right face states; sl, sr : slopes of the left/right cells.
+// Returns the branch taken: 0 left-supersonic, 1 right-supersonic,
+// 2 star region with u* <= 0, 3 star region with u* > 0.
+template <bool Exact>
+FVM_HD int grp(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Gas &G, IfState &W,
+               IfState &Wt);
+
+template <>
+FVM_HD int grp<true>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Gas &G,
+                     IfState &W, IfState &Wt) {
+    const double gamma = G.gamma;
+    double rho_l = fl.r, rho_r = fr.r, pi_l = fl.p, pi_r = fr.p;
+    double u_l = fl.un, u_r = fr.un, v_l = fl.ut, v_r = fr.ut;
+    double e_l = pi_l / rho_l / (gamma - 1.0);
+    double e_r = pi_r / rho_r / (gamma - 1.0);
+    double c_l = 0.0, c_r = 0.0;
+    double rho_lx = sl.r, rho_rx = sr.r, pi_lx = sl.p, pi_rx = sr.p;
+    double u_lx = sl.un, u_rx = sr.un, v_lx = sl.ut, v_rx = sr.ut;
+    double e_lx = pi_lx / rho_l / (gamma - 1.0) - pi_l * rho_lx / rho_l / rho_l / (gamma - 1.0);
+    double e_rx = pi_rx / rho_r / (gamma - 1.0) - pi_r * rho_rx / rho_r / rho_r / (gamma - 1.0);
+    double c_lx = 0.5 * sqrt(gamma / pi_l / rho_l) * (pi_l * rho_lx + rho_l * pi_lx);
+    double c_rx = 0.5 * sqrt(gamma / pi_r / rho_r) * (pi_r * rho_rx + rho_r * pi_rx);
+    double rho = 0.0, pi = 0.0, u = 0.0, v = 0.0, e = 0.0;
+    double rho_t = 0.0, pi_t = 0.0, u_t = 0.0, v_t = 0.0, e_t = 0.0;
+    double a_l = sqrt(gamma * pi_l / rho_l), a_r = sqrt(gamma * pi_r / rho_r);
+    double kappa = (gamma + 1.0) / 2.0;
+    if (pi_r >= pi_l) {
+        double c_l_ = rho_l * (a_l + kappa * tmax(0.0, (pi_r - pi_l) / rho_r / a_r + u_l - u_r));
+        c_l = c_l_;
+        c_r = rho_r * (a_r + kappa * tmax(0.0, (pi_l - pi_r) / c_l_ + u_l - u_r));
+    } else {
+        double c_r_ = rho_r * (a_r + kappa * tmax(0.0, (pi_l - pi_r) / rho_l / a_l + u_l - u_r));
+        double c_l_ = rho_l * (a_l + kappa * tmax(0.0, (pi_r - pi_l) / c_r_ + u_l - u_r));
+        c_l = c_l_;
+        c_r = c_r_;
+    }
+    int br;
+    if (u_l - c_l / rho_l >= 0.0) {
+        br = 0;
+        rho = rho_l, pi = pi_l, u = u_l, v = v_l, e = e_l;
+        rho_t = -rho * u_lx - u * rho_lx;
+        u_t = -u * u_lx - pi_lx / rho;
+        e_t = -u * e_lx - pi / rho * u_lx;
+        v_t = -u * v_lx;
+        pi_t = -u * pi_lx - c_l * c_l / rho_l * u_lx;
+    } else if (u_r + c_r / rho_r <= 0.0) {
+        br = 1;
+        rho = rho_r, pi = pi_r, u = u_r, v = v_r, e = e_r;
+        rho_t = -rho * u_rx - u * rho_rx;
+        u_t = -u * u_rx - pi_rx / rho;
+        e_t = -u * e_rx - pi / rho * u_rx;
+        v_t = -u * v_rx;
+        pi_t = -u * pi_rx - c_r * c_r / rho * u_rx;
+    } else {
+        double u_m = (u_l * c_l + u_r * c_r + pi_l - pi_r) / (c_l + c_r);
+        double pi_m = ((u_l - u_r) * c_l * c_r + c_l * pi_r + c_r * pi_l) / (c_l + c_r);
+        // u is still 0.0 in these four (src/basicFuns.cuh:110,151-154)
+        double k_l = -pi_lx - c_l * u_lx + 0.5 * (u - u_l) * c_lx;
+        double k_r = -pi_rx + c_r * u_rx - 0.5 * (u - u_r) * c_rx;
+        double d_l = -2.0 * c_l * c_l / rho_l * u_lx - 2.0 * c_l / rho_l * pi_lx +
+                     c_l / rho_l * (u - u_l) * c_lx;
+        double d_r = 2.0 * c_r * c_r / rho_r * u_rx - 2.0 * c_r / rho_r * pi_rx +
+                     c_r / rho_r * (u_r - u) * c_rx;
+        double Dpi_Dt = (c_r * d_l - c_l * d_r) / 2.0 / (c_l + c_r);
+        pi = pi_m;
+        u = u_m;
+        if (u <= 0.0) {
+            br = 2;
+            double rho_mr = c_r / (u_r - u_m + c_r / rho_r);
+            double e_mr = e_r + (pi_m - pi_r) * (pi_m + pi_r) / 2.0 / c_r / c_r;
+            rho = rho_mr;
+            e = e_mr;
+            v = v_r;
+            rho_t = -u * rho * rho * rho / (rho_r * c_r * c_r) *
+                        (c_r * c_r / rho_r / rho_r * rho_rx - c_r * u_rx + 1.5 * (u - u_r) * c_rx) +
+                    rho * rho * rho / (c_r * c_r * c_r) * (u + c_r / rho) * Dpi_Dt;
+            pi_t = 1.0 / (c_l + c_r) *
+                   (c_l * c_r / rho_l * (1 + u * rho / c_r) * k_l -
+                    c_l * c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            u_t = 1.0 / (c_l + c_r) *
+                  (c_l / rho_l * (1 + u * rho / c_r) * k_l +
+                   c_r / rho_r * (1 - u * rho * c_l / c_r / c_r) * k_r);
+            e_t = -u * rho / rho_r / c_r / c_r / c_r *
+                      (-pi_r * c_r * pi_rx + c_r * c_r * c_r * e_rx +
+                       2.0 * (e_r - e) * c_r * c_r * c_rx) +
+                  pi / c_r / c_r * pi_t;
+            double v_x = rho / rho_r * v_rx;
+            v_t = -u * v_x;
+        } else {
+            br = 3;
+            double rho_ml = c_l / (u_m - u_l + c_l / rho_l);
+            double e_ml = e_l + (pi_m - pi_l) * (pi_m + pi_l) / 2.0 / c_l / c_l;
+            rho = rho_ml;
+            e = e_ml;
+            v = v_l;
+            rho_t = -u * rho * rho * rho / (rho_l * c_l * c_l) *
+                        (c_l * c_l / rho_l / rho_l * rho_lx + c_l * u_lx + 1.5 * (u - u_l) * c_lx) -
+                    rho * rho * rho / (c_l * c_l * c_l) * (u - c_l / rho) * Dpi_Dt;
+            pi_t = 1.0 / (c_l + c_r) *
+                   (c_l * c_r / rho_l * (1 + u * rho / c_r) * k_l -
+                    c_l * c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            u_t = 1.0 / (c_l + c_r) *
+                  (c_l / rho_l * (1 + u * rho * c_r / c_l / c_l) * k_l +
+                   c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            e_t = -u * rho / rho_l / c_l / c_l / c_l *
+                      (-pi_l * c_l * pi_lx + c_l * c_l * c_l * e_lx +
+                       2.0 * (e_l - e) * c_l * c_l * c_lx) +
+                  pi / c_l / c_l * pi_t;
+            double v_x = rho / rho_l * v_lx;
+            v_t = -u * v_x;
+        }
+    }
+    W = IfState{rho, pi, u, v, e};
+    Wt = IfState{rho_t, pi_t, u_t, v_t, e_t};
+    return br;
+}
+
+// Reciprocal for the fast path (correctly rounded division by default).
+FVM_HD double rcp(double x) { return 1.0 / x; }
+
+template <>
+FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Gas &G,
+                      IfState &W, IfState &Wt) {
+    const double gamma = G.gamma, g1i = G.g1i, kappa = G.kappa;
+    const double rl = fl.r, rr = fr.r, pl = fl.p, pr = fr.p;
+    const double ul = fl.un, ur = fr.un;
+    const double irl = rcp(rl), irr = rcp(rr);
+    const double el = pl * irl * g1i, er = pr * irr * g1i;
+    // e_x = (p_x - p rho_x / rho) / rho / (gamma - 1)
+    const double elx = fma(-pl * irl, sl.r, sl.p) * irl * g1i;
+    const double erx = fma(-pr * irr, sr.r, sr.p) * irr * g1i;
+    // Lagrangian sound speeds S = rho a = sqrt(gamma p rho)
+    const double Sl = sqrt(gamma * pl * rl), Sr = sqrt(gamma * pr * rr);
+    const double iSl = rcp(Sl), iSr = rcp(Sr);
+    // c_x = 0.5 sqrt(gamma/(p rho)) (p rho_x + rho p_x) = 0.5 gamma/S (...)
+    const double clx = 0.5 * gamma * iSl * fma(pl, sl.r, rl * sl.p);
+    const double crx = 0.5 * gamma * iSr * fma(pr, sr.r, rr * sr.p);
+    const double du = ul - ur;
+    double cl, cr, icl, icr;
+    if (pr >= pl) {
+        cl = fma(rl * kappa, tmax(0.0, fma(pr - pl, iSr, du)), Sl);
+        icl = rcp(cl);
+        cr = fma(rr * kappa, tmax(0.0, fma(pl - pr, icl, du)), Sr);
+        icr = rcp(cr);
+    } else {
+        cr = fma(rr * kappa, tmax(0.0, fma(pl - pr, iSl, du)), Sr);
+        icr = rcp(cr);
+        cl = fma(rl * kappa, tmax(0.0, fma(pr - pl, icr, du)), Sl);
+        icl = rcp(cl);
+    }
+    if (ul - cl * irl >= 0.0) {
+        const double u = ul;
+        W = IfState{rl, pl, ul, fl.ut, el};
+        Wt.r = -fma(rl, sl.un, u * sl.r);
+        Wt.un = -fma(u, sl.un, sl.p * irl);
+        Wt.e = -fma(u, elx, pl * irl * sl.un);
+        Wt.ut = -u * sl.ut;
+        Wt.p = -fma(u, sl.p, cl * cl * irl * sl.un);
+        return 0;
+    }
+    if (ur + cr * irr <= 0.0) {
+        const double u = ur;
+        W = IfState{rr, pr, ur, fr.ut, er};
+        Wt.r = -fma(rr, sr.un, u * sr.r);
+        Wt.un = -fma(u, sr.un, sr.p * irr);
+        Wt.e = -fma(u, erx, pr * irr * sr.un);
+        Wt.ut = -u * sr.ut;
+        Wt.p = -fma(u, sr.p, cr * cr * irr * sr.un);
+        return 1;
+    }
+    const double ics = rcp(cl + cr);
+    const double um = (fma(ul, cl, fma(ur, cr, pl - pr))) * ics;
+    const double pm = fma(du * cl, cr, fma(cl, pr, cr * pl)) * ics;
+    // k and d use u = 0 (see the exact form)
+    const double kl = fma(-0.5 * ul, clx, fma(-cl, sl.un, -sl.p));
+    const double kr = fma(0.5 * ur, crx, fma(cr, sr.un, -sr.p));
+    const double dl = cl * irl * fma(-2.0 * cl, sl.un, fma(-ul, clx, -2.0 * sl.p));
+    const double dr = cr * irr * fma(2.0 * cr, sr.un, fma(ur, crx, -2.0 * sr.p));
+    const double Dp = fma(cr, dl, -cl * dr) * 0.5 * ics;
+    if (um <= 0.0) {
+        const double den = (ur - um) + cr * irr;
+        const double rho = cr * rcp(den);
+        const double irho = den * icr;
+        const double e = fma((pm - pr) * (pm + pr), 0.5 * icr * icr, er);
+        const double r3 = rho * rho * rho;
+        const double icr2 = icr * icr;
+        const double t1 = fma(cr * irr * irr, cr * sr.r, fma(-cr, sr.un, 1.5 * (um - ur) * crx));
+        const double rho_t = fma(-um * r3 * irr * icr2, t1, r3 * icr2 * icr * fma(cr, irho, um) * Dp);
+        const double urho = um * rho;
+        const double A = cl * irl * fma(urho, icr, 1.0) * kl;
+        const double B = cr * irr * fma(-urho, icl, 1.0) * kr;
+        const double p_t = ics * (cr * A - cl * B);
+        const double u_t = ics * fma(cr * irr * fma(-urho * cl, icr2, 1.0), kr, A);
+        const double e_t = fma(-urho * irr, fma(2.0 * (er - e) * icr, crx, fma(-pr * icr2, sr.p, erx)),
+                               pm * icr2 * p_t);
+        W = IfState{rho, pm, um, fr.ut, e};
+        Wt = IfState{rho_t, p_t, u_t, -um * (rho * irr * sr.ut), e_t};
+        return 2;
+    }
+    const double den = (um - ul) + cl * irl;
+    const double rho = cl * rcp(den);
+    const double irho = den * icl;
+    const double e = fma((pm - pl) * (pm + pl), 0.5 * icl * icl, el);
+    const double r3 = rho * rho * rho;
+    const double icl2 = icl * icl;
+    const double t1 = fma(cl * irl * irl, cl * sl.r, fma(cl, sl.un, 1.5 * (um - ul) * clx));
+    const double rho_t = fma(-um * r3 * irl * icl2, t1, -r3 * icl2 * icl * fma(-cl, irho, um) * Dp);
+    const double urho = um * rho;
+    const double A = cl * irl * fma(urho, icr, 1.0) * kl;
+    const double B = cr * irr * fma(-urho, icl, 1.0) * kr;
+    const double p_t = ics * (cr * A - cl * B);
+    const double u_t = ics * fma(cl * irl * fma(urho * cr, icl2, 1.0), kl, B);
+    const double e_t = fma(-urho * irl, fma(2.0 * (el - e) * icl, clx, fma(-pl * icl2, sl.p, elx)),
+                           pm * icl2 * p_t);
+    W = IfState{rho, pm, um, fl.ut, e};
+    Wt = IfState{rho_t, p_t, u_t, -um * (rho * irl * sl.ut), e_t};
+    return 3;
+}
+
+// ---------------------------------------------------------------- flux
+// F = f(W) + (k/2) d f(W)/dt in the order (mass, normal momentum, energy,
+// tangential momentum), src/basicFuns.cuh:14-34; the energy flux uses the
+// GRP's e, not p/((gamma-1) rho).
+template <bool Exact>
+FVM_HD Q flux(const IfState &U, const IfState &Ut, double k) {
+    const double rho = U.r, pi = U.p, u = U.un, v = U.ut, e = U.e;
+    const double rho_t = Ut.r, pi_t = Ut.p, u_t = Ut.un, v_t = Ut.ut, e_t = Ut.e;
+    if (Exact) {
+        return Q{rho * u + k / 2.0 * (rho * u_t + u * rho_t),
+                 rho * u * u + pi + k / 2.0 * (rho_t * u * u + 2.0 * rho * u * u_t + pi_t),
+                 rho * u * (e + u * u / 2.0 + v * v / 2.0) + pi * u +
+                     k / 2.0 *
+                         (rho_t * u * e + rho * u_t * e + rho * u * e_t + rho_t * u * u * u / 2.0 +
+                          1.5 * rho * u * u * u_t + rho_t * u * v * v / 2.0 + rho * u * v * v_t +
+                          rho * u_t * v * v / 2.0 + pi_t * u + pi * u_t),
+                 rho * u * v + k / 2.0 * (rho * u_t * v + u * rho_t * v + u * rho * v_t)};
+    }
+    const double kh = 0.5 * k;
+    const double ru = rho * u;
+    const double K = fma(0.5 * u, u, fma(0.5 * v, v, e));       // e + u^2/2 + v^2/2
+    const double mt = fma(rho, u_t, u * rho_t);                   // (rho u)_t
+    const double f0 = fma(kh, mt, ru);
+    const double f1 = fma(kh, fma(rho_t * u, u, fma(2.0 * ru, u_t, pi_t)), fma(ru, u, pi));
+    const double g2 = fma(rho_t * u, K, fma(rho * u_t, K + u * u, fma(ru, fma(v, v_t, e_t),
+                                                                        fma(pi_t, u, pi * u_t))));
+    const double f2 = fma(kh, g2, fma(ru, K, pi * u));
+    const double f3 = fma(kh, fma(v, mt, ru * v_t), ru * v);
+    return Q{f0, f1, f2, f3};
+}
+
+// ---------------------------------------------------------------- update
+// src/basicFuns.cuh:229-246 with src/basicFuns.cuh:195-212: convert to
+// conserved (rho, rho un, E, rho ut), subtract lambda (F_right - F_left),
+// convert back to primitive.
+template <bool Exact>
+FVM_HD Q update(const Q &q, const Q &FL, const Q &FR, double lambda, const Gas &G) {
+    if (Exact) {
+        const double gamma = G.gamma;
+        double c0 = q.r, c1 = q.r * q.un,
+               c2 = q.p / (gamma - 1.0) + q.r * (q.un * q.un + q.ut * q.ut) / 2.0, c3 = q.r * q.ut;
+        c0 = c0 - (FR.r - FL.r) * lambda;
+        c1 = c1 - (FR.p - FL.p) * lambda;
+        c2 = c2 - (FR.un - FL.un) * lambda;
+        c3 = c3 - (FR.ut - FL.ut) * lambda;
+        return Q{c0, (gamma - 1.0) * (c2 - c1 * c1 / c0 / 2.0 - c3 * c3 / c0 / 2.0), c1 / c0, c3 / c0};
+    }
+    const double c0 = fma(-lambda, FR.r - FL.r, q.r);
+    const double c1 = fma(-lambda, FR.p - FL.p, q.r * q.un);
+    const double E = fma(q.p, G.g1i, 0.5 * q.r * fma(q.un, q.un, q.ut * q.ut));
+    const double c2 = fma(-lambda, FR.un - FL.un, E);
+    const double c3 = fma(-lambda, FR.ut - FL.ut, q.r * q.ut);
+    const double ir = rcp(c0);
+    const double un = c1 * ir, ut = c3 * ir;
+    const double p = G.g1 * fma(-0.5 * ir, fma(c1, c1, c3 * c3), c2);
+    return Q{c0, p, un, ut};
+}
+
+// src/basicFuns.cuh:223-224: |(u, v)| + sqrt(gamma p / rho).
+template <bool Exact>
+FVM_HD double speed(double rho, double p, double u, double v, const Gas &G) {
+    if (Exact) {
+        double velo = sqrt(u * u + v * v), c = sqrt(G.gamma * p / rho);
+        return fabs(velo) + c;
+    }
+    return sqrt(fma(u, u, v * v)) + sqrt(G.gamma * p / rho);
+}
+
+}  // namespace fvm2d

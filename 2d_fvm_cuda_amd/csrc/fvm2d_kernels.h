@@ -1,0 +1,17 @@
+// Host-side launchers of the step kernels (defined in fvm2d_step.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fvm2d_device.h"
+
+namespace fvm2d {
+
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks, int ntasks,
+                 Sched *sched, int persistent, int grid, int wpb, hipStream_t stream);
+void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
+                  long long ncells, hipStream_t stream);
+void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, hipStream_t stream);
+int step_kernel_occupancy(bool exact, int wpb);
+
+}  // namespace fvm2d

@@ -1,0 +1,447 @@
+// Fused split-step kernel for the 2D relaxation-GRP Euler solver (gfx950).
+//
+// One launch advances every active instance by one full time step of the
+// reference's scheme (src/wrappers.cuh:196-284):
+//     A1 = X(U), A = Y(A1)      branch "X then Y"
+//     B1 = Y(U), B = X(B1)      branch "Y then X"
+//     U' = fill((A + B) / 2)    primitive average + ghost preset at t + dt
+//     s_max(U') -> dt of the next step
+// A wave owns a window of 64 consecutive columns (60 output columns plus a
+// two-column halo on each side) and marches down a strip of R output rows.
+// Both Y-sweeps are marched in registers along the rows (no transposes, no
+// LDS); both X-sweeps exchange neighbour data across lanes with DPP
+// wave_shr/wave_shl.  A1 and B1 never leave registers, so HBM sees one read
+// of U (plus the halo rows) and one write of U'.
+//
+// The ghost preset (src/ghostCellPresets.cuh:47-241, two-phase row-then-
+// column order) is applied by the lane that owns each ghost cell's source
+// cell; constant inflow states are written by the clamped owner and enter
+// the wave-speed maximum directly.  The maximum of the new state is reduced
+// per wave (64-bit bit-pattern max, NaN sorts high), combined with one
+// atomic per wave, and the last wave of an instance advances t, dt and the
+// step counter on the device, so the host never waits inside the time loop.
+#include <hip/hip_runtime.h>
+
+#include "fvm2d_device.h"
+#include "fvm2d_kernels.h"
+
+namespace fvm2d {
+
+// ---------------------------------------------------------------- lanes
+__device__ __forceinline__ double dpp_shr1(double x) {  // lane l <- lane l-1
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double dpp_shl1(double x) {  // lane l <- lane l+1
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ Q shr(const Q &q) {
+    return Q{dpp_shr1(q.r), dpp_shr1(q.p), dpp_shr1(q.un), dpp_shr1(q.ut)};
+}
+__device__ __forceinline__ Q shl(const Q &q) {
+    return Q{dpp_shl1(q.r), dpp_shl1(q.p), dpp_shl1(q.un), dpp_shl1(q.ut)};
+}
+
+// X-order (rho, p, u, v) <-> Y-line order (rho, p, v, u)
+__device__ __forceinline__ Q swap_uv(const Q &q) { return Q{q.r, q.p, q.ut, q.un}; }
+
+// ---------------------------------------------------------------- sweeps
+// One X-sweep of a row held across the wave, lanes = consecutive columns.
+// Output is valid on lanes 2..61 (a two-lane stencil halo each side).
+template <bool Exact>
+__device__ __forceinline__ Q xsweep(const Q &q, const Line &L, const Gas &G, double dt,
+                                    double lam) {
+    const Q qL = shr(q), qR = shl(q);
+    const Q s = slope<Exact>(qL, q, qR, L);
+    const Q fm = face_minus<Exact>(q, s, L);
+    const Q fp = face_plus<Exact>(q, s, L);
+    const Q fpL = shr(fp), sL = shr(s);
+    IfState W, Wt;
+    grp<Exact>(fpL, fm, sL, s, G, W, Wt);  // interface on the west face
+    const Q F = flux<Exact>(W, Wt, dt);
+    const Q FR = shl(F);                   // interface on the east face
+    return update<Exact>(q, F, FR, lam, G);
+}
+
+// Register pipeline of a Y-sweep marching south (increasing row j).  In the
+// sweep's line order the "left" neighbour of row j is row j+1, which is the
+// orientation the reference's rotate_and_flip_vertically produces.
+struct YMarch {
+    Q qj, qj1;   // rows j and j+1 (line order)
+    Q sj, fmj;   // slope and south face of row j
+    Q Fprev;     // flux through the north face of row j
+};
+
+template <bool Exact>
+__device__ __forceinline__ void ymarch_init(YMarch &m, const Q &m2, const Q &m1, const Q &q0,
+                                            const Q &p1, const Line &L, const Gas &G, double dt) {
+    // rows r0-2, r0-1, r0, r0+1
+    const Q s_m1 = slope<Exact>(q0, m1, m2, L);
+    const Q s_0 = slope<Exact>(p1, q0, m1, L);
+    const Q fm_m1 = face_minus<Exact>(m1, s_m1, L);
+    const Q fp_0 = face_plus<Exact>(q0, s_0, L);
+    IfState W, Wt;
+    grp<Exact>(fp_0, fm_m1, s_0, s_m1, G, W, Wt);
+    m.Fprev = flux<Exact>(W, Wt, dt);
+    m.qj = q0;
+    m.qj1 = p1;
+    m.sj = s_0;
+    m.fmj = face_minus<Exact>(q0, s_0, L);
+}
+
+// Consume row j+2, return the updated row j.
+template <bool Exact>
+__device__ __forceinline__ Q ymarch_step(YMarch &m, const Q &qn, const Line &L, const Gas &G,
+                                         double dt, double lam) {
+    const Q s1 = slope<Exact>(qn, m.qj1, m.qj, L);
+    const Q fp1 = face_plus<Exact>(m.qj1, s1, L);
+    const Q fm1 = face_minus<Exact>(m.qj1, s1, L);
+    IfState W, Wt;
+    grp<Exact>(fp1, m.fmj, s1, m.sj, G, W, Wt);
+    const Q F = flux<Exact>(W, Wt, dt);               // south face of row j
+    const Q out = update<Exact>(m.qj, F, m.Fprev, lam, G);
+    m.qj = m.qj1;
+    m.qj1 = qn;
+    m.sj = s1;
+    m.fmj = fm1;
+    m.Fprev = F;
+    return out;
+}
+
+// ---------------------------------------------------------------- ghosts
+struct GhostSrc {
+    int is_const;
+    int sj, si;
+    int negv;
+    double r, p, u, v;
+};
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// Value of ghost cell (gj, gi) after the preset's two-phase fill, as a copy
+// of an interior cell (optionally with v negated) or a constant state.
+__device__ GhostSrc ghost_source(int strategy, int gj, int gi, int nx, int ny, double t) {
+    GhostSrc g{0, clampi(gj, 2, ny - 3), clampi(gi, 2, nx - 3), 0, 0.0, 0.0, 0.0, 0.0};
+    auto set_const = [&](double r, double p, double u, double v) {
+        g.is_const = 1;
+        g.r = r;
+        g.p = p;
+        g.u = u;
+        g.v = v;
+    };
+    if (strategy == kOutflow) return g;  // ghostCellPresets.cuh:106-145
+    if (strategy == kJet) {              // ghostCellPresets.cuh:148-193
+        if (gi <= 1) {
+            if (gj >= 2 * ny / 5 + 1 && gj <= 3 * ny / 5) set_const(5.0, 0.4127, 800.0, 0.0);
+            else set_const(0.5, 0.4127, 0.0, 0.0);
+        }
+        return g;
+    }
+    if (strategy == kJet800) {  // ghostCellPresets.cuh:196-241
+        if (gi <= 1) {
+            if (gj >= 9 * ny / 10) set_const(1.4, 1.0, 800.0, 0.0);
+            else set_const(0.14, 1.0, 0.0, 0.0);
+        } else if (gj == ny - 1) {
+            g.sj = ny - 4;
+        }
+        return g;
+    }
+    // double Mach (also selected by "reflective"), ghostCellPresets.cuh:48-103
+    const double post_u = 4.125 * sqrt(3.0);
+    if (gi <= 1) {
+        set_const(8.0, 116.5, post_u, -4.125);
+    } else if (gi >= nx - 2) {
+        set_const(1.4, 1.0, 0.0, 0.0);
+    } else if (gj <= 1) {
+        double temp = double(ny) / sqrt(3.0);
+        double dist = double(ny) * 20 * t / sqrt(3.0);
+        if (gi <= nx / 24 + int(temp) + int(dist)) set_const(8.0, 116.5, post_u, -4.125);
+        else set_const(1.4, 1.0, 0.0, 0.0);
+    } else if (gj == ny - 2) {
+        g.sj = ny - 3;
+        g.negv = gi > nx / 24;
+    } else {  // gj == ny - 1
+        if (gi <= nx / 24) {
+            g.sj = ny - 3;
+        } else {
+            g.sj = ny - 4;
+            g.negv = 1;
+        }
+    }
+    return g;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+
+__device__ __forceinline__ void store4(double *base, long long plane, long long off, double r,
+                                       double p, double u, double v) {
+    base[off] = r;
+    base[plane + off] = p;
+    base[2 * plane + off] = u;
+    base[3 * plane + off] = v;
+}
+
+// Write every ghost cell whose fill source (or, for constants, whose
+// clamped owner) is interior cell (j, c) holding value (r, p, u, v).
+template <bool Exact>
+__device__ void emit_ghosts(const InstDesc &d, double *dst, int j, int c, double r, double p,
+                            double u, double v, double t_new, unsigned long long &mx) {
+    const int nx = d.nx, ny = d.ny;
+    const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
+    const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
+    for (int a = 0; a < 5; a++) {
+        const int gj = rows[a];
+        for (int b = 0; b < 5; b++) {
+            const int gi = cols[b];
+            if (a == 0 && b == 0) continue;
+            const bool ghost = gj < 2 || gj > ny - 3 || gi < 2 || gi > nx - 3;
+            if (!ghost) continue;
+            const GhostSrc g = ghost_source(d.strategy, gj, gi, nx, ny, t_new);
+            const long long off = (long long)gj * d.pitch + gi;
+            if (g.is_const) {
+                if (clampi(gj, 2, ny - 3) == j && clampi(gi, 2, nx - 3) == c) {
+                    store4(dst, d.plane, off, g.r, g.p, g.u, g.v);
+                    const unsigned long long sb = dbits(speed<Exact>(g.r, g.p, g.u, g.v, d.gas));
+                    mx = sb > mx ? sb : mx;
+                }
+            } else if (g.sj == j && g.si == c) {
+                store4(dst, d.plane, off, r, p, u, g.negv ? -v : v);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long y = __shfl_xor(x, o, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// dt rule of src/wrappers.cuh:202-207 evaluated on the device.
+__device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_new, int steps_new,
+                                            unsigned long long sbits) {
+    const double smax = __longlong_as_double((long long)sbits);
+    const double h = d.hmin;
+    const double a = d.cfl * h / smax, b = d.t_end - t_new;
+    const double dt = b < a ? b : a;  // std::min(a, b)
+    int done = !(t_new < d.t_end);
+    if (d.max_steps >= 0 && steps_new >= d.max_steps) done = 1;
+    if (h / smax < 1e-20) done = 1;
+    int status = 0;
+    if (sbits > 0x7ff0000000000000ull) {  // NaN (either sign) sorts above +inf
+        status = 1;
+        done = 1;
+    }
+    c->t = t_new;
+    c->dt = dt;
+    c->smax = smax;
+    c->steps = steps_new;
+    c->done = done;
+    c->status |= status;
+    c->smax_next = 0ull;
+    c->waves_done = 0u;
+}
+
+// ---------------------------------------------------------------- the step
+template <bool Exact>
+__device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
+    const int nx = d.nx, ny = d.ny;
+    const long long P = d.plane;
+    const double t = ctl->t, dt = ctl->dt;
+    const int steps = ctl->steps;
+    const double *__restrict__ src = (steps & 1) ? d.buf1 : d.buf0;
+    double *__restrict__ dst = (steps & 1) ? d.buf0 : d.buf1;
+    const Gas G = d.gas;
+    const Line LX = d.lx, LY = d.ly;
+    const double lamx = dt / LX.h, lamy = dt / LY.h;
+    const double t_new = t + dt;
+
+    const int w = local % d.nw, s = local / d.nw;
+    const int col = w * kOwn + lane;
+    const int colc = col < nx ? col : nx - 1;
+    const int r0 = 2 + s * d.R;
+    const int r1 = min(ny - 2, r0 + d.R);
+    const bool own = lane >= 2 && lane < kWave - 2 && col <= nx - 3;
+
+    auto load = [&](int j) -> Q {  // X-order
+        const long long o = (long long)j * d.pitch + colc;
+        return Q{src[o], src[P + o], src[2 * P + o], src[3 * P + o]};
+    };
+
+    // prologue: rows r0-2 .. r0+1
+    const Q u_m2 = load(r0 - 2), u_m1 = load(r0 - 1), u_0 = load(r0), u_p1 = load(r0 + 1);
+    YMarch mu, ma;
+    ymarch_init<Exact>(mu, swap_uv(u_m2), swap_uv(u_m1), swap_uv(u_0), swap_uv(u_p1), LY, G, dt);
+    {
+        const Q a_m2 = xsweep<Exact>(u_m2, LX, G, dt, lamx);
+        const Q a_m1 = xsweep<Exact>(u_m1, LX, G, dt, lamx);
+        const Q a_0 = xsweep<Exact>(u_0, LX, G, dt, lamx);
+        const Q a_p1 = xsweep<Exact>(u_p1, LX, G, dt, lamx);
+        ymarch_init<Exact>(ma, swap_uv(a_m2), swap_uv(a_m1), swap_uv(a_0), swap_uv(a_p1), LY, G,
+                           dt);
+    }
+
+    unsigned long long mx = 0ull;
+    Q u_n = load(r0 + 2);
+    for (int j = r0; j < r1; j++) {
+        const Q u_cur = u_n;
+        if (j + 1 < r1) u_n = load(j + 3);  // prefetch the next row
+        // branch B, first sweep: B1 row j
+        const Q b1 = ymarch_step<Exact>(mu, swap_uv(u_cur), LY, G, dt, lamy);
+        // branch A, first sweep on the look-ahead row: A1 row j+2
+        const Q a1 = xsweep<Exact>(u_cur, LX, G, dt, lamx);
+        // branch A, second sweep: A row j
+        const Q a = swap_uv(ymarch_step<Exact>(ma, swap_uv(a1), LY, G, dt, lamy));
+        // branch B, second sweep: B row j
+        const Q b = xsweep<Exact>(swap_uv(b1), LX, G, dt, lamx);
+        // src/basicFuns.cuh:264-274 average on primitives
+        const double r = (a.r + b.r) * 0.5, p = (a.p + b.p) * 0.5;
+        const double u = (a.un + b.un) * 0.5, v = (a.ut + b.ut) * 0.5;
+        if (own) {
+            const long long o = (long long)j * d.pitch + col;
+            store4(dst, P, o, r, p, u, v);
+            const unsigned long long sb = dbits(speed<Exact>(r, p, u, v, G));
+            mx = sb > mx ? sb : mx;
+            if (j == 2 || j == ny - 4 || j == ny - 3 || col == 2 || col == nx - 3)
+                emit_ghosts<Exact>(d, dst, j, col, r, p, u, v, t_new, mx);
+        }
+    }
+
+    mx = wave_max_u64(mx);
+    if (lane == 0) {
+        atomicMax(&ctl->smax_next, mx);
+        __threadfence();
+        const unsigned int prev = atomicAdd(&ctl->waves_done, 1u);
+        if (prev == (unsigned int)(d.nw * d.ns - 1)) {
+            __threadfence();
+            const unsigned long long sb = atomicOr(&ctl->smax_next, 0ull);
+            advance_ctl(d, ctl, t_new, steps + 1, sb);
+        }
+    }
+}
+
+template <bool Exact>
+__global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restrict__ descs,
+                                                       Ctl *__restrict__ ctls,
+                                                       const int2 *__restrict__ tasks, int ntasks,
+                                                       Sched *sched, int persistent) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int wpb = blockDim.x / kWave;
+    int tk;
+    if (persistent) {
+        unsigned int v = 0;
+        if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
+        tk = __builtin_amdgcn_readfirstlane((int)__shfl(v, 0, kWave));
+    } else {
+        tk = blockIdx.x * wpb + wid;
+    }
+    while (tk < ntasks) {
+        const int2 task = tasks[tk];
+        const int inst = __builtin_amdgcn_readfirstlane(task.x);
+        const int local = __builtin_amdgcn_readfirstlane(task.y);
+        Ctl *c = ctls + inst;
+        if (!c->done) step_task<Exact>(descs[inst], c, local, lane);
+        if (!persistent) break;
+        unsigned int v = 0;
+        if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
+        tk = __builtin_amdgcn_readfirstlane((int)__shfl(v, 0, kWave));
+    }
+    if (persistent && lane == 0) {
+        const unsigned int total = gridDim.x * wpb;
+        const unsigned int e = atomicAdd(&sched->exited, 1u);
+        if (e == total - 1) {  // every wave has left the task loop
+            atomicExch(&sched->ticket, 0u);
+            atomicExch(&sched->exited, 0u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- set-up
+// s_max over every cell of the current state (ghosts included), as
+// maxPropagtingSpeed + thrust::reduce do at src/wrappers.cuh:198-201.
+template <bool Exact>
+__global__ void __launch_bounds__(256) fvm_speed_kernel(const InstDesc *__restrict__ descs,
+                                                        Ctl *__restrict__ ctls, int inst,
+                                                        int use_buf1) {
+    const InstDesc d = descs[inst];
+    const double *src = use_buf1 ? d.buf1 : d.buf0;
+    const long long n = (long long)d.nx * d.ny;
+    unsigned long long mx = 0ull;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        const double sp = speed<Exact>(src[o], src[d.plane + o], src[2 * d.plane + o],
+                                       src[3 * d.plane + o], d.gas);
+        const unsigned long long b = dbits(sp);
+        mx = b > mx ? b : mx;
+    }
+    mx = wave_max_u64(mx);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&ctls[inst].smax_next, mx);
+}
+
+__global__ void fvm_ctl_start_kernel(const InstDesc *__restrict__ descs, Ctl *__restrict__ ctls,
+                                     int ninst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    Ctl *c = ctls + i;
+    const unsigned long long sb = c->smax_next;
+    // the reference's loop test happens before the first step: t = 0
+    const InstDesc d = descs[i];
+    c->status = 0;
+    advance_ctl(d, c, 0.0, 0, sb);
+}
+
+// ---------------------------------------------------------------- launchers
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks, int ntasks,
+                 Sched *sched, int persistent, int grid, int wpb, hipStream_t stream) {
+    const dim3 block(wpb * kWave);
+    if (exact)
+        hipLaunchKernelGGL(fvm_step_kernel<true>, dim3(grid), block, 0, stream, descs, ctls, tasks,
+                           ntasks, sched, persistent);
+    else
+        hipLaunchKernelGGL(fvm_step_kernel<false>, dim3(grid), block, 0, stream, descs, ctls, tasks,
+                           ntasks, sched, persistent);
+}
+
+void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
+                  long long ncells, hipStream_t stream) {
+    long long blocks = (ncells + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    if (exact)
+        hipLaunchKernelGGL(fvm_speed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           descs, ctls, inst, use_buf1);
+    else
+        hipLaunchKernelGGL(fvm_speed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           descs, ctls, inst, use_buf1);
+}
+
+void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, hipStream_t stream) {
+    hipLaunchKernelGGL(fvm_ctl_start_kernel, dim3((ninst + 63) / 64), dim3(64), 0, stream, descs,
+                       ctls, ninst);
+}
+
+int step_kernel_occupancy(bool exact, int wpb) {
+    int blocks = 0;
+    hipError_t e;
+    if (exact)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fvm_step_kernel<true>,
+                                                         wpb * kWave, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fvm_step_kernel<false>,
+                                                         wpb * kWave, 0);
+    return e == hipSuccess ? blocks : 0;
+}
+
+}  // namespace fvm2d

@@ -1,0 +1,136 @@
+/*
+ * fvm2d: MI355X (gfx950) 2D relaxation-GRP finite-volume Euler solver.
+ * C ABI of libfvm2d.so.  Plain pointers and sizes only.
+ *
+ * State layout at this boundary is the reference's: an array of vec5d
+ * records, [ny][nx][5] doubles, row 0 = north, column 0 = west, primitive
+ * (rho, p, u, v, unused); the outer two rows/columns are ghost cells
+ * (src/ghostCellPresets.cuh:4-45, src/wrappers.cuh:67-78, README.md:4).
+ *
+ * Every function returns 0 on success or a negative FVM2D_ERR_* code; the
+ * message of the last failure on the calling thread is fvm2d_last_error().
+ * The reference checks no return codes; here bad arguments, HIP failures
+ * and a non-finite wave speed (NaN state) are reported.
+ */
+#ifndef FVM2D_H
+#define FVM2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVM2D_OK 0
+#define FVM2D_ERR_ARG (-1)
+#define FVM2D_ERR_HIP (-2)
+#define FVM2D_ERR_NONFINITE (-3)
+#define FVM2D_ERR_NODEVICE (-4)
+#define FVM2D_ERR_STATE (-5)
+
+/* enum class ghostCellStrategy, src/wrappers.cuh:17-24 (same order).
+ * REFLECTIVE runs the double-Mach preset, as src/wrappers.cuh:277-280 does. */
+enum {
+    FVM2D_OUTFLOW = 0,
+    FVM2D_REFLECTIVE = 1,
+    FVM2D_JET = 2,
+    FVM2D_JET800 = 3,
+    FVM2D_DOUBLE_MACH = 4
+};
+
+/* Arithmetic mode. FAST shares reciprocals and uses FMA; EXACT evaluates the
+ * reference's expressions in its order without contraction and reproduces
+ * the FP64 host restatement bit for bit. */
+enum { FVM2D_MODE_FAST = 0, FVM2D_MODE_EXACT = 1 };
+
+/* The solver's settable parameters: RGRP_2D_CUDA's setters
+ * (src/wrappers.cuh:110-152) and defaults (src/wrappers.cuh:49-52). */
+typedef struct {
+    double width;   /* setSpatialLayout, default 1 */
+    double height;  /* setSpatialLayout, default 1 */
+    double t_end;   /* setTimeLayout, default 0 */
+    double cfl;     /* setCFL / setTimeLayout, default 0.5 */
+    double alpha;   /* setAlpha, default 1.9 */
+    double gamma;   /* setGamma, default 1.4 */
+    int strategy;   /* setGhostCellStrategy, default FVM2D_OUTFLOW */
+    int max_steps;  /* extension: < 0 runs until t >= t_end (reference behaviour) */
+    int mode;       /* FVM2D_MODE_FAST (default) or FVM2D_MODE_EXACT */
+    int reserved;
+} fvm2d_params;
+
+/* Fill p with the reference's defaults (src/wrappers.cuh:49-52). */
+void fvm2d_params_default(fvm2d_params *p);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *fvm2d_last_error(void);
+
+/* 1 if a HIP device is usable, else 0. */
+int fvm2d_device_available(void);
+
+/* RGRP_2D_CUDA::solve (src/wrappers.cuh:154-306): blocking; advances U5 in
+ * place from t = 0 to t_end (or max_steps), leaves the final state with its
+ * ghost cells filled.  steps_out: time steps taken; ms_out: milliseconds of
+ * the device time loop (the region of src/wrappers.cuh:193-288). */
+int fvm2d_solve(double *U5, int nx, int ny, const fvm2d_params *p, int *steps_out,
+                double *ms_out);
+
+/* ---- resident solver: the state stays in HBM between calls ------------- */
+typedef struct fvm2d_solver fvm2d_solver;
+
+/* RGRP_2D_CUDA(U0) (src/wrappers.cuh:67-78) split in create + upload. */
+int fvm2d_create(int nx, int ny, const fvm2d_params *p, fvm2d_solver **out);
+int fvm2d_set_params(fvm2d_solver *s, const fvm2d_params *p);
+int fvm2d_upload(fvm2d_solver *s, const double *U5);
+/* Device-to-device copy in/out of a contiguous [4][ny][nx] plane array
+ * (rho, p, u, v) that lives in device memory. */
+int fvm2d_upload_planes_device(fvm2d_solver *s, const double *d_planes);
+int fvm2d_download_planes_device(fvm2d_solver *s, double *d_planes);
+/* The solve loop of src/wrappers.cuh:191-288 on the resident state: t
+ * restarts at 0, the state continues from the last one (T11 of the
+ * reference: a second solve() resumes from the previous result). */
+int fvm2d_run(fvm2d_solver *s, int *steps_out, double *t_out, double *ms_out);
+/* solve() copies the final state out: src/wrappers.cuh:290-291. */
+int fvm2d_download(fvm2d_solver *s, double *U5);
+/* Total time steps over all runs (getTimeStagesCount, src/wrappers.cuh:313-316). */
+int fvm2d_steps_total(const fvm2d_solver *s);
+int fvm2d_destroy(fvm2d_solver *s);
+
+/* Launch geometry of the step kernel.  strip_rows: output rows per wave;
+ * waves_per_block: 1..4; persistent: 1 = persistent grid with dynamic task
+ * tickets.  0 / negative values keep the current choice. */
+int fvm2d_set_launch(fvm2d_solver *s, int strip_rows, int waves_per_block, int persistent);
+int fvm2d_get_launch(const fvm2d_solver *s, int *strip_rows, int *waves_per_block,
+                     int *persistent);
+/* Time every candidate launch geometry on a scratch copy of the state and
+ * keep the fastest; the state is left unchanged.  best_ms_out: per step. */
+int fvm2d_autotune(fvm2d_solver *s, int steps_per_trial, double *best_ms_out);
+/* Start a time loop at t = 0 without running it, then advance at most
+ * nsteps steps (asynchronously); fvm2d_sync waits and reports. */
+int fvm2d_begin(fvm2d_solver *s);
+int fvm2d_advance(fvm2d_solver *s, int nsteps);
+int fvm2d_sync(fvm2d_solver *s, int *steps_out, double *t_out, double *dt_out);
+/* Wall time of nsteps launches bracketed by HIP events on the solver's
+ * stream (after fvm2d_begin).  ms_out: total milliseconds. */
+int fvm2d_time_steps(fvm2d_solver *s, int nsteps, double *ms_out);
+
+/* ---- batched forward (no counterpart in the reference) ----------------- *
+ * n independent meshes of their own size and parameters advance together:
+ * one persistent launch per time step covers every instance's tiles, each
+ * instance keeps its own t, dt and step counter on the device, and the
+ * step loop is captured in a hipGraph. */
+typedef struct fvm2d_batch fvm2d_batch;
+int fvm2d_batch_create(int n, const int *nx, const int *ny, const fvm2d_params *p,
+                       fvm2d_batch **out);
+int fvm2d_batch_upload(fvm2d_batch *b, int i, const double *U5);
+int fvm2d_batch_begin(fvm2d_batch *b);
+/* Launch up to nsteps steps (captured once per chunk size in a hipGraph);
+ * returns when every instance is done or nsteps launches were issued. */
+int fvm2d_batch_run(fvm2d_batch *b, int nsteps, double *ms_out);
+int fvm2d_batch_status(fvm2d_batch *b, int i, int *steps_out, double *t_out);
+int fvm2d_batch_download(fvm2d_batch *b, int i, double *U5);
+int fvm2d_batch_set_launch(fvm2d_batch *b, int strip_rows, int waves_per_block, int persistent);
+int fvm2d_batch_autotune(fvm2d_batch *b, int steps_per_trial, double *best_ms_out);
+int fvm2d_batch_destroy(fvm2d_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM2D_H */

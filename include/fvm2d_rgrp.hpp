@@ -1,0 +1,149 @@
+// Drop-in C++ front end with the reference's class interface, forwarding to
+// the C ABI of libfvm2d.so (include/fvm2d.h).  A caller written against the
+// reference's src/wrappers.cuh (e.g. test2D.cu's main) compiles with a host
+// compiler and links -lfvm2d; no CUDA/HIP toolchain is needed on its side.
+//
+//   vec5d              <- src/ghostCellPresets.cuh:4-45 (same 40-byte layout,
+//                         +, - and *scalar on slots 0..3, slot 4 -> 0)
+//   ghostCellStrategy  <- src/wrappers.cuh:17-24 (same enumerators, same order)
+//   RGRP_2D_CUDA       <- src/wrappers.cuh:26-65 (same methods and defaults)
+//
+// Visualisation (setRecordModel's PNG frames, src/wrappers.cuh:323-345) needs
+// OpenCV, which this build does not use: recording only stores the settings.
+#pragma once
+
+#include <cstddef>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fvm2d.h"
+
+struct vec5d {
+    vec5d() : data{0.0, 0.0, 0.0, 0.0, 0.0} {}
+    vec5d(double x0, double x1, double x2, double x3, double x4) : data{x0, x1, x2, x3, x4} {}
+    double &operator[](size_t i) { return data[i]; }
+    const double &operator[](size_t i) const { return data[i]; }
+    double &operator()(size_t i) { return data[i]; }
+    const double &operator()(size_t i) const { return data[i]; }
+    vec5d operator+(const vec5d &v) const {
+        vec5d r;
+        for (int i = 0; i < 4; i++) r.data[i] = data[i] + v.data[i];
+        return r;
+    }
+    vec5d operator-(const vec5d &v) const {
+        vec5d r;
+        for (int i = 0; i < 4; i++) r.data[i] = data[i] - v.data[i];
+        return r;
+    }
+    vec5d operator*(double a) const {
+        vec5d r;
+        for (int i = 0; i < 4; i++) r.data[i] = data[i] * a;
+        return r;
+    }
+    double data[5];
+};
+static_assert(sizeof(vec5d) == 5 * sizeof(double), "vec5d must be five packed doubles");
+
+enum class ghostCellStrategy { outflow, reflective, jet, jet800, doubleMach };
+
+class RGRP_2D_CUDA {
+  public:
+    explicit RGRP_2D_CUDA(const std::vector<std::vector<vec5d>> &U0) {
+        for (auto &row : U0)
+            for (auto &u : row) U0_.push_back(u);
+        Nx_ = (int)U0[0].size();
+        Ny_ = (int)U0.size();
+    }
+    RGRP_2D_CUDA() {}
+
+    void setRPInitialData(const vec5d &U1, const vec5d &U2, const vec5d &U3, const vec5d &U4, int Nx,
+                          int Ny) {
+        U0_.assign((size_t)Nx * Ny, vec5d());
+        Nx_ = Nx;
+        Ny_ = Ny;
+        for (int j = 0; j < Ny; ++j)
+            for (int i = 0; i < Nx; ++i) {
+                const bool west = i < Nx / 2, north = j < Ny / 2;
+                U0_[(size_t)j * Nx + i] = west ? (north ? U2 : U3) : (north ? U1 : U4);
+            }
+    }
+    void setSpatialLayout(double width, double height) {
+        width_ = width;
+        height_ = height;
+    }
+    void setTimeLayout(double endTime, double CFL) {
+        endTime_ = endTime;
+        CFL_ = CFL;
+    }
+    void setTimeLayout(double endTime) { endTime_ = endTime; }
+    void setCFL(double CFL) { CFL_ = CFL; }
+    void setAlpha(double alpha) { alpha_ = alpha; }
+    void setGamma(double gamma) { gamma_ = gamma; }
+    void setGhostCellStrategy(ghostCellStrategy s) { strategy_ = s; }
+    void setRecordModel(bool on_off, int gapPerFrame = 10, const std::string &recordDir = ".") {
+        record_ = on_off;
+        gap_ = gapPerFrame;
+        recordDir_ = recordDir;
+    }
+    // Extension: FVM2D_MODE_FAST (default) or FVM2D_MODE_EXACT.
+    void setArithmeticMode(int mode) { mode_ = mode; }
+
+    std::vector<std::vector<vec5d>> solve() {
+        fvm2d_params p;
+        fvm2d_params_default(&p);
+        p.width = width_;
+        p.height = height_;
+        p.t_end = endTime_;
+        p.cfl = CFL_;
+        p.alpha = alpha_;
+        p.gamma = gamma_;
+        p.strategy = (int)strategy_;
+        p.mode = mode_;
+        int steps = 0;
+        double ms = 0.0;
+        const int rc = fvm2d_solve(reinterpret_cast<double *>(U0_.data()), Nx_, Ny_, &p, &steps, &ms);
+        if (rc != FVM2D_OK) throw std::runtime_error(std::string("fvm2d_solve: ") + fvm2d_last_error());
+        steps_ += steps;
+        ms_ = ms;
+        U_ = vectorized();
+        return U_;
+    }
+    double getComputationTime_ms() { return ms_; }
+    int getTimeStagesCount() { return steps_; }
+    void writeFinalResultTo_txt(const std::string &dir) {
+        const char *names[4] = {"rho", "p", "u", "v"};
+        for (int k = 0; k < 4; k++) {
+            std::ofstream f(dir + "/" + names[k] + "_" + std::to_string(endTime_) + "s.txt");
+            for (const auto &row : U_) {
+                for (const auto &e : row) f << e(k) << " ";
+                f << "\n";
+            }
+        }
+    }
+    void clear() {
+        U0_.clear();
+        U_.clear();
+    }
+
+  protected:
+    std::vector<std::vector<vec5d>> vectorized() const {
+        std::vector<std::vector<vec5d>> r(Ny_);
+        for (int j = 0; j < Ny_; ++j)
+            r[j].assign(U0_.begin() + (size_t)j * Nx_, U0_.begin() + (size_t)(j + 1) * Nx_);
+        return r;
+    }
+    std::vector<vec5d> U0_;
+    std::vector<std::vector<vec5d>> U_;
+    double width_ = 1.0, height_ = 1.0, endTime_ = 0.0, CFL_ = 0.5;
+    double alpha_ = 1.9, gamma_ = 1.4;
+    int Nx_ = 100, Ny_ = 100;
+    ghostCellStrategy strategy_ = ghostCellStrategy::outflow;
+    bool record_ = false;
+    int gap_ = 1;
+    std::string recordDir_ = ".";
+    double ms_ = 0.0;
+    int steps_ = 0;
+    int mode_ = FVM2D_MODE_FAST;
+};

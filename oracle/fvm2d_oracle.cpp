@@ -1,0 +1,488 @@
+// FP64 host oracle: statement-for-statement restatement of the reference's
+// solve path.  TEST INFRASTRUCTURE ONLY (the checker, never the product).
+// Each routine cites the reference lines it follows; expression trees keep
+// the reference's left-to-right evaluation order so that, compiled with
+// -ffp-contract=off, every result is bit-reproducible.
+#include "fvm2d_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ghostCellPresets.cuh:4-45 -- five doubles; +, - and *scalar act on slots
+// 0..3 only and leave slot 4 of the result at 0.
+struct V5 {
+    double d[5];
+    V5() : d{0.0, 0.0, 0.0, 0.0, 0.0} {}
+    V5(double a, double b, double c, double e, double f) : d{a, b, c, e, f} {}
+    double &operator[](int i) { return d[i]; }
+    const double &operator[](int i) const { return d[i]; }
+    V5 operator+(const V5 &v) const {
+        V5 r;
+        for (int i = 0; i < 4; i++) r.d[i] = d[i] + v.d[i];
+        return r;
+    }
+    V5 operator-(const V5 &v) const {
+        V5 r;
+        for (int i = 0; i < 4; i++) r.d[i] = d[i] - v.d[i];
+        return r;
+    }
+    V5 operator*(double a) const {
+        V5 r;
+        for (int i = 0; i < 4; i++) r.d[i] = d[i] * a;
+        return r;
+    }
+};
+static_assert(sizeof(V5) == 40, "vec5d layout");
+
+// thrust::min / thrust::max as used at basicFuns.cuh:62,117-124
+inline double tmin(double a, double b) { return b < a ? b : a; }
+inline double tmax(double a, double b) { return a < b ? b : a; }
+
+// basicFuns.cuh:14-34
+V5 flux(const V5 &U, const V5 &U_t, double k) {
+    double rho = U[0], pi = U[1], u = U[2], v = U[3], e = U[4];
+    double rho_t = U_t[0], pi_t = U_t[1], u_t = U_t[2], v_t = U_t[3], e_t = U_t[4];
+    return V5(
+        rho * u + k / 2.0 * (rho * u_t + u * rho_t),
+        rho * u * u + pi + k / 2.0 * (rho_t * u * u + 2.0 * rho * u * u_t + pi_t),
+        rho * u * (e + u * u / 2.0 + v * v / 2.0) + pi * u +
+            k / 2.0 * (rho_t * u * e + rho * u_t * e + rho * u * e_t + rho_t * u * u * u / 2.0 +
+                       1.5 * rho * u * u * u_t + rho_t * u * v * v / 2.0 + rho * u * v * v_t +
+                       rho * u_t * v * v / 2.0 + pi_t * u + pi * u_t),
+        rho * u * v + k / 2.0 * (rho * u_t * v + u * rho_t * v + u * rho * v_t),
+        0.0);
+}
+
+// basicFuns.cuh:49-68 for one cell: limited slope of cell "c" from its line
+// neighbours L and R.
+V5 slope(const V5 &UL, const V5 &Uc, const V5 &UR, double h, double alpha) {
+    V5 Ul_s = (UR - UL) * 0.5 * (1.0 / h);
+    V5 U_s = (Uc - UL) * (1.0 / h) * alpha;
+    V5 Ur_s = (UR - Uc) * (1.0 / h) * alpha;
+    V5 Us;  // slot 4 never written by the reference kernel (zero-initialised buffer)
+    for (int i = 0; i <= 3; i++) {
+        double sgn13 = Ul_s[i] * Ur_s[i], sgn23 = U_s[i] * Ur_s[i];
+        if (sgn13 > 0.0 && sgn23 > 0.0) {
+            double sgn = 1.0;
+            if (Ul_s[i] <= 0.0) sgn = -1.0;
+            Us[i] = sgn * tmin(std::fabs(Ul_s[i]), tmin(std::fabs(U_s[i]), std::fabs(Ur_s[i])));
+        } else {
+            Us[i] = 0.0;
+        }
+    }
+    return Us;
+}
+
+// basicFuns.cuh:86-189 for one interface.  fl = left state (face of the
+// left cell), fr = right state, sl/sr = slopes of the left/right cells.
+// branch: 0 left supersonic, 1 right supersonic, 2 star u*<=0, 3 star u*>0.
+void grp(const V5 &fl, const V5 &fr, const V5 &sl, const V5 &sr, double gamma, V5 &W,
+         V5 &Wt, int *branch) {
+    double rho_l = fl[0];
+    double rho_r = fr[0];
+    double pi_l = fl[1];
+    double pi_r = fr[1];
+    double u_l = fl[2];
+    double u_r = fr[2];
+    double v_l = fl[3];
+    double v_r = fr[3];
+    double e_l = pi_l / rho_l / (gamma - 1.0);
+    double e_r = pi_r / rho_r / (gamma - 1.0);
+    double c_l = 0.0;
+    double c_r = 0.0;
+    double rho_lx = sl[0];
+    double rho_rx = sr[0];
+    double pi_lx = sl[1];
+    double pi_rx = sr[1];
+    double u_lx = sl[2];
+    double u_rx = sr[2];
+    double v_lx = sl[3];
+    double v_rx = sr[3];
+    double e_lx = pi_lx / rho_l / (gamma - 1.0) - pi_l * rho_lx / rho_l / rho_l / (gamma - 1.0);
+    double e_rx = pi_rx / rho_r / (gamma - 1.0) - pi_r * rho_rx / rho_r / rho_r / (gamma - 1.0);
+    double c_lx = 0.5 * std::sqrt(gamma / pi_l / rho_l) * (pi_l * rho_lx + rho_l * pi_lx);
+    double c_rx = 0.5 * std::sqrt(gamma / pi_r / rho_r) * (pi_r * rho_rx + rho_r * pi_rx);
+    double rho = 0.0, pi = 0.0, u = 0.0, v = 0.0, e = 0.0;
+    double rho_t = 0.0, pi_t = 0.0, u_t = 0.0, v_t = 0.0, e_t = 0.0;
+
+    double a_l = std::sqrt(gamma * pi_l / rho_l), a_r = std::sqrt(gamma * pi_r / rho_r);
+    double kappa = (gamma + 1.0) / 2.0;
+    if (pi_r >= pi_l) {
+        double c_l_ = rho_l * (a_l + kappa * tmax(0.0, (pi_r - pi_l) / rho_r / a_r + u_l - u_r));
+        c_l = c_l_;
+        c_r = rho_r * (a_r + kappa * tmax(0.0, (pi_l - pi_r) / c_l_ + u_l - u_r));
+    } else {
+        double c_r_ = rho_r * (a_r + kappa * tmax(0.0, (pi_l - pi_r) / rho_l / a_l + u_l - u_r));
+        double c_l_ = rho_l * (a_l + kappa * tmax(0.0, (pi_r - pi_l) / c_r_ + u_l - u_r));
+        c_l = c_l_;
+        c_r = c_r_;
+    }
+
+    int br;
+    if (u_l - c_l / rho_l >= 0.0) {
+        br = 0;
+        rho = rho_l, pi = pi_l, u = u_l, v = v_l, e = e_l;
+        rho_t = -rho * u_lx - u * rho_lx;
+        u_t = -u * u_lx - pi_lx / rho;
+        e_t = -u * e_lx - pi / rho * u_lx;
+        v_t = -u * v_lx;
+        pi_t = -u * pi_lx - c_l * c_l / rho_l * u_lx;
+    } else if (u_r + c_r / rho_r <= 0.0) {
+        br = 1;
+        rho = rho_r, pi = pi_r, u = u_r, v = v_r, e = e_r;
+        rho_t = -rho * u_rx - u * rho_rx;
+        u_t = -u * u_rx - pi_rx / rho;
+        e_t = -u * e_rx - pi / rho * u_rx;
+        v_t = -u * v_rx;
+        pi_t = -u * pi_rx - c_r * c_r / rho * u_rx;
+    } else {
+        double u_m = (u_l * c_l + u_r * c_r + pi_l - pi_r) / (c_l + c_r);
+        double pi_m = ((u_l - u_r) * c_l * c_r + c_l * pi_r + c_r * pi_l) / (c_l + c_r);
+        // u is still 0.0 here (basicFuns.cuh:110,151-154)
+        double k_l = -pi_lx - c_l * u_lx + 0.5 * (u - u_l) * c_lx;
+        double k_r = -pi_rx + c_r * u_rx - 0.5 * (u - u_r) * c_rx;
+        double d_l = -2.0 * c_l * c_l / rho_l * u_lx - 2.0 * c_l / rho_l * pi_lx + c_l / rho_l * (u - u_l) * c_lx;
+        double d_r = 2.0 * c_r * c_r / rho_r * u_rx - 2.0 * c_r / rho_r * pi_rx + c_r / rho_r * (u_r - u) * c_rx;
+        double Dpi_Dt = (c_r * d_l - c_l * d_r) / 2.0 / (c_l + c_r);
+        pi = pi_m;
+        u = u_m;
+        if (u <= 0.0) {
+            br = 2;
+            double rho_mr = c_r / (u_r - u_m + c_r / rho_r);
+            double e_mr = e_r + (pi_m - pi_r) * (pi_m + pi_r) / 2.0 / c_r / c_r;
+            rho = rho_mr;
+            e = e_mr;
+            v = v_r;
+            rho_t = -u * rho * rho * rho / (rho_r * c_r * c_r) *
+                        (c_r * c_r / rho_r / rho_r * rho_rx - c_r * u_rx + 1.5 * (u - u_r) * c_rx) +
+                    rho * rho * rho / (c_r * c_r * c_r) * (u + c_r / rho) * Dpi_Dt;
+            pi_t = 1.0 / (c_l + c_r) *
+                   (c_l * c_r / rho_l * (1 + u * rho / c_r) * k_l - c_l * c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            u_t = 1.0 / (c_l + c_r) *
+                  (c_l / rho_l * (1 + u * rho / c_r) * k_l + c_r / rho_r * (1 - u * rho * c_l / c_r / c_r) * k_r);
+            e_t = -u * rho / rho_r / c_r / c_r / c_r *
+                      (-pi_r * c_r * pi_rx + c_r * c_r * c_r * e_rx + 2.0 * (e_r - e) * c_r * c_r * c_rx) +
+                  pi / c_r / c_r * pi_t;
+            double v_x = rho / rho_r * v_rx;
+            v_t = -u * v_x;
+        } else {
+            br = 3;
+            double rho_ml = c_l / (u_m - u_l + c_l / rho_l);
+            double e_ml = e_l + (pi_m - pi_l) * (pi_m + pi_l) / 2.0 / c_l / c_l;
+            rho = rho_ml;
+            e = e_ml;
+            v = v_l;
+            rho_t = -u * rho * rho * rho / (rho_l * c_l * c_l) *
+                        (c_l * c_l / rho_l / rho_l * rho_lx + c_l * u_lx + 1.5 * (u - u_l) * c_lx) -
+                    rho * rho * rho / (c_l * c_l * c_l) * (u - c_l / rho) * Dpi_Dt;
+            pi_t = 1.0 / (c_l + c_r) *
+                   (c_l * c_r / rho_l * (1 + u * rho / c_r) * k_l - c_l * c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            u_t = 1.0 / (c_l + c_r) *
+                  (c_l / rho_l * (1 + u * rho * c_r / c_l / c_l) * k_l + c_r / rho_r * (1 - u * rho / c_l) * k_r);
+            e_t = -u * rho / rho_l / c_l / c_l / c_l *
+                      (-pi_l * c_l * pi_lx + c_l * c_l * c_l * e_lx + 2.0 * (e_l - e) * c_l * c_l * c_lx) +
+                  pi / c_l / c_l * pi_t;
+            double v_x = rho / rho_l * v_lx;
+            v_t = -u * v_x;
+        }
+    }
+    W = V5{rho, pi, u, v, e};
+    Wt = V5(rho_t, pi_t, u_t, v_t, e_t);
+    if (branch) *branch = br;
+}
+
+// basicFuns.cuh:195-202
+V5 rhoPUVToConserVar(const V5 &q, double gamma) {
+    return {q[0], q[0] * q[2], q[1] / (gamma - 1.0) + q[0] * (q[2] * q[2] + q[3] * q[3]) / 2.0,
+            q[0] * q[3], 0.0};
+}
+
+// basicFuns.cuh:205-212
+V5 conserVarToRhoPUV(const V5 &U_, double gamma) {
+    return {U_[0], (gamma - 1.0) * (U_[2] - U_[1] * U_[1] / U_[0] / 2.0 - U_[3] * U_[3] / U_[0] / 2.0),
+            U_[1] / U_[0], U_[3] / U_[0], 0.0};
+}
+
+// basicFuns.cuh:223-224
+double speed_of(const V5 &q, double gamma) {
+    double velo = std::sqrt(q[2] * q[2] + q[3] * q[3]), rho = q[0], p = q[1],
+           c = std::sqrt(gamma * p / rho);
+    return std::fabs(velo) + c;
+}
+
+// ---- whole-array kernels (the reference's __global__ functions as loops) ----
+
+// basicFuns.cuh:37-72: cells i in [1, Nx-2] of every row.
+void k_muscl(const V5 *U, V5 *Us, int Nx, int Ny, double alpha, double width, V5 *Ulr) {
+    double h = width / double(Nx);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 1; i < Nx - 1; i++) {
+            int idxL = j * Nx + i - 1, idx = j * Nx + i, idxR = j * Nx + i + 1;
+            Us[idx] = slope(U[idxL], U[idx], U[idxR], h, alpha);
+            Ulr[2 * idx + 1] = U[idx] - Us[idx] * h * 0.5;
+            Ulr[2 * idx + 2] = U[idx] + Us[idx] * h * 0.5;
+        }
+    }
+}
+
+// basicFuns.cuh:75-192: interfaces i in [2, Nx-2] of every row.
+void k_grp(const V5 *Us, const V5 *Ulr, int Nx, int Ny, V5 *uut, double gamma) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 2; i < Nx - 1; i++) {
+            int idxL = j * Nx + i - 1, idx = j * Nx + i;
+            V5 W, Wt;
+            grp(Ulr[2 * idx], Ulr[2 * idx + 1], Us[idxL], Us[idx], gamma, W, Wt, nullptr);
+            uut[2 * idx] = W;
+            uut[2 * idx + 1] = Wt;
+        }
+    }
+}
+
+// basicFuns.cuh:229-246: cells i in [2, Nx-3] of every row, in place.
+void k_forward_x(V5 *U, const V5 *uut, int Nx, int Ny, double width, double t, double gamma) {
+    double h = width / double(Nx);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 2; i < Nx - 2; i++) {
+            int idx = j * Nx + i;
+            V5 F_L = flux(uut[2 * idx], uut[2 * idx + 1], t);
+            V5 F_R = flux(uut[2 * idx + 2], uut[2 * idx + 3], t);
+            V5 U_ = rhoPUVToConserVar(U[idx], gamma);
+            double lambda = t / h;
+            U_ = U_ - (F_R - F_L) * lambda;
+            U[idx] = conserVarToRhoPUV(U_, gamma);
+        }
+    }
+}
+
+// basicFuns.cuh:249-262
+void k_rotate(const V5 *in, V5 *out, int Nx, int Ny) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < Ny; y++) {
+        for (int x = 0; x < Nx; x++) {
+            int outputCol = Ny - 1 - y;
+            int outputRow = Nx - 1 - x;
+            int outputIdx = outputRow * Ny + outputCol;
+            out[outputIdx] = in[y * Nx + x];
+            std::swap(out[outputIdx].d[2], out[outputIdx].d[3]);
+        }
+    }
+}
+
+// basicFuns.cuh:264-274
+void k_combine(V5 *U1, const V5 *U2, int n) {
+#pragma omp parallel for schedule(static)
+    for (int idx = 0; idx < n; idx++) U1[idx] = (U1[idx] + U2[idx]) * 0.5;
+}
+
+// ghostCellPresets.cuh:47-241.  Every preset writes its ghost rows in the
+// first branch and its ghost columns in the second; the column branch of a
+// corner reads values that the row branch of another thread wrote.  On the
+// reference GPU the row branch of the whole warp runs first, so the fill is
+// restated as two phases: all row writes, then all column writes.
+const double SQRT3 = std::sqrt(3.0);
+
+void k_fill(V5 *U, int Nx, int Ny, int strategy, double width, double height, double t) {
+    (void)width;
+    (void)height;
+    const V5 post2M{8, 116.5, 4.125 * SQRT3, -4.125, 0};
+    const V5 pre2M{1.4, 1, 0, 0, 0};
+    // phase 1: ghost rows
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 0; i < Nx; i++) {
+            int idx = j * Nx + i;
+            if (strategy == 0 || strategy == 2 || strategy == 3) {
+                if (j == 0) U[idx] = U[idx + 2 * Nx];
+                else if (j == 1) U[idx] = U[idx + Nx];
+                else if (j == Ny - 2) U[idx] = U[idx - Nx];
+                else if (j == Ny - 1) U[idx] = (strategy == 3) ? U[idx - 3 * Nx] : U[idx - 2 * Nx];
+            } else {  // reflective and doubleMach both run the double-Mach preset (wrappers.cuh:277-280)
+                if (j == 0 || j == 1) {
+                    double temp = double(Ny) / std::sqrt(3.0);
+                    double dist = double(Ny) * 20 * t / std::sqrt(3.0);
+                    if (i <= Nx / 24 + int(temp) + int(dist)) U[idx] = post2M;
+                    else U[idx] = pre2M;
+                } else if (j == Ny - 1) {
+                    if (i <= Nx / 24) {
+                        U[idx] = U[idx - 2 * Nx];
+                    } else {
+                        U[idx] = U[idx - 3 * Nx];
+                        U[idx][3] = -U[idx][3];
+                    }
+                } else if (j == Ny - 2) {
+                    if (i <= Nx / 24) {
+                        U[idx] = U[idx - Nx];
+                    } else {
+                        U[idx] = U[idx - Nx];
+                        U[idx][3] = -U[idx][3];
+                    }
+                }
+            }
+        }
+    }
+    // phase 2: ghost columns
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 0; i < Nx; i++) {
+            int idx = j * Nx + i;
+            if (strategy == 0) {
+                if (i == 0) U[idx] = U[idx + 2];
+                else if (i == 1) U[idx] = U[idx + 1];
+                else if (i == Nx - 2) U[idx] = U[idx - 1];
+                else if (i == Nx - 1) U[idx] = U[idx - 2];
+            } else if (strategy == 2) {
+                if (i == 0 || i == 1) {
+                    if (j >= 2 * Ny / 5 + 1 && j <= 3 * Ny / 5) U[idx] = V5(5.0, 0.4127, 800.0, 0.0, 0.0);
+                    else U[idx] = {0.5, 0.4127, 0.0, 0.0, 0.0};
+                } else if (i == Nx - 2) U[idx] = U[idx - 1];
+                else if (i == Nx - 1) U[idx] = U[idx - 2];
+            } else if (strategy == 3) {
+                if (i == 0 || i == 1) {
+                    if (j >= 9 * Ny / 10) U[idx] = V5(1.4, 1, 800, 0, 0);
+                    else U[idx] = {0.14, 1, 0, 0, 0};
+                } else if (i == Nx - 2) U[idx] = U[idx - 1];
+                else if (i == Nx - 1) U[idx] = U[idx - 2];
+            } else {
+                if (i == 0 || i == 1) U[idx] = post2M;
+                else if (i == Nx - 2 || i == Nx - 1) U[idx] = pre2M;
+            }
+        }
+    }
+}
+
+// One X-direction operator on an Nx x Ny array (wrappers.cuh:211-215).
+void sweep_x(V5 *U, V5 *Us, V5 *Ulr, V5 *uut, int Nx, int Ny, double width, double dt,
+             double alpha, double gamma) {
+    k_muscl(U, Us, Nx, Ny, alpha, width, Ulr);
+    k_grp(Us, Ulr, Nx, Ny, uut, gamma);
+    k_forward_x(U, uut, Nx, Ny, width, dt, gamma);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads, int *steps_out,
+              double *t_out, double *ms_out) {
+    if (!U5 || !p || nx < 1 || ny < 1) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int Nx = nx, Ny = ny;
+    const size_t N = size_t(Nx) * Ny;
+    V5 *u = reinterpret_cast<V5 *>(U5);
+    std::vector<V5> ucopy(N), uy(N), us(N), usy(N), ulr(2 * (N + Ny)), ulry(2 * (N + Nx)),
+        uut(2 * (N + Ny)), uuty(2 * (N + Nx));
+    const double W = p->width, H = p->height, CFL = p->cfl, al = p->alpha, g = p->gamma;
+    double cTime = 0.0;
+    int steps = 0;
+    auto start = std::chrono::steady_clock::now();
+    // wrappers.cuh:196-284
+    while (cTime < p->t_end) {
+        if (p->max_steps >= 0 && steps >= p->max_steps) break;
+        double max_speed = -std::numeric_limits<double>::infinity();
+#pragma omp parallel for reduction(max : max_speed) schedule(static)
+        for (long long idx = 0; idx < (long long)N; idx++) {
+            double s = speed_of(u[idx], g);
+            max_speed = tmax(max_speed, s);
+        }
+        double h = std::min(W / double(Nx), H / double(Ny));
+        double dt = std::min(CFL * h / max_speed, p->t_end - cTime);
+        if (h / max_speed < 1e-20) break;
+
+        std::memcpy(ucopy.data(), u, N * sizeof(V5));
+        // branch A: X then Y
+        sweep_x(u, us.data(), ulr.data(), uut.data(), Nx, Ny, W, dt, al, g);
+        k_rotate(u, uy.data(), Nx, Ny);
+        sweep_x(uy.data(), usy.data(), ulry.data(), uuty.data(), Ny, Nx, H, dt, al, g);
+        k_rotate(uy.data(), u, Ny, Nx);
+        // branch B: Y then X
+        k_rotate(ucopy.data(), uy.data(), Nx, Ny);
+        sweep_x(uy.data(), usy.data(), ulry.data(), uuty.data(), Ny, Nx, H, dt, al, g);
+        k_rotate(uy.data(), ucopy.data(), Ny, Nx);
+        sweep_x(ucopy.data(), us.data(), ulr.data(), uut.data(), Nx, Ny, W, dt, al, g);
+
+        k_combine(u, ucopy.data(), (int)N);
+        cTime += dt;
+        steps++;
+        k_fill(u, Nx, Ny, p->strategy, W, H, cTime);
+    }
+    auto end = std::chrono::steady_clock::now();
+    if (steps_out) *steps_out = steps;
+    if (t_out) *t_out = cTime;
+    if (ms_out) *ms_out = std::chrono::duration<double, std::milli>(end - start).count();
+    return 0;
+}
+
+void orc_slope(const double *qL, const double *q, const double *qR, double h, double alpha,
+               double *sigma4) {
+    V5 a(qL[0], qL[1], qL[2], qL[3], 0), b(q[0], q[1], q[2], q[3], 0), c(qR[0], qR[1], qR[2], qR[3], 0);
+    V5 s = slope(a, b, c, h, alpha);
+    for (int i = 0; i < 4; i++) sigma4[i] = s[i];
+}
+
+void orc_grp(const double *fl, const double *fr, const double *sl, const double *sr, double gamma,
+             double *W5, double *Wt5, int *branch) {
+    V5 a(fl[0], fl[1], fl[2], fl[3], 0), b(fr[0], fr[1], fr[2], fr[3], 0);
+    V5 c(sl[0], sl[1], sl[2], sl[3], 0), d(sr[0], sr[1], sr[2], sr[3], 0);
+    V5 W, Wt;
+    grp(a, b, c, d, gamma, W, Wt, branch);
+    for (int i = 0; i < 5; i++) {
+        W5[i] = W[i];
+        Wt5[i] = Wt[i];
+    }
+}
+
+void orc_flux(const double *W5, const double *Wt5, double k, double *F4) {
+    V5 a(W5[0], W5[1], W5[2], W5[3], W5[4]), b(Wt5[0], Wt5[1], Wt5[2], Wt5[3], Wt5[4]);
+    V5 F = flux(a, b, k);
+    for (int i = 0; i < 4; i++) F4[i] = F[i];
+}
+
+void orc_cons(const double *q, double gamma, double *c4) {
+    V5 r = rhoPUVToConserVar(V5(q[0], q[1], q[2], q[3], 0), gamma);
+    for (int i = 0; i < 4; i++) c4[i] = r[i];
+}
+
+void orc_prim(const double *c, double gamma, double *q4) {
+    V5 r = conserVarToRhoPUV(V5(c[0], c[1], c[2], c[3], 0), gamma);
+    for (int i = 0; i < 4; i++) q4[i] = r[i];
+}
+
+double orc_speed(const double *q, double gamma) { return speed_of(V5(q[0], q[1], q[2], q[3], 0), gamma); }
+
+void orc_fill(double *U5, int nx, int ny, int strategy, double width, double height, double t) {
+    k_fill(reinterpret_cast<V5 *>(U5), nx, ny, strategy, width, height, t);
+}
+
+void orc_sweep_x(double *U5, int nx, int ny, double width, double dt, double alpha, double gamma,
+                 int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    size_t N = size_t(nx) * ny;
+    std::vector<V5> us(N), ulr(2 * (N + ny)), uut(2 * (N + ny));
+    sweep_x(reinterpret_cast<V5 *>(U5), us.data(), ulr.data(), uut.data(), nx, ny, width, dt, alpha, gamma);
+}
+
+void orc_rotate(const double *in5, double *out5, int nx, int ny) {
+    k_rotate(reinterpret_cast<const V5 *>(in5), reinterpret_cast<V5 *>(out5), nx, ny);
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+"""GPU parity of the fused step kernel against the FP64 host oracle.
+
+EXACT mode (reference op order, no FMA contraction) must reproduce the oracle
+bit for bit; FAST mode (shared reciprocals, FMA) must stay within relative
+L1 <= 1e-10 on (rho, rho u, rho v, E) -- the tolerance the plain solve is
+held to.  All calls go through the C ABI of libfvm2d.so.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL_L1 = 1e-10
+
+
+def _cases(problems):
+    rng = problems.SplitMix64(7)
+    cases = []
+    # (name, U0, params)
+    cases.append(("4shocks64", problems.initialize_4shocks(64), dict(t_end=0.05)))
+    cases.append(("rand37x23", problems.random_four_quadrant(rng, 37, 23), dict(t_end=1.0, max_steps=6)))
+    cases.append(("rand6x6", problems.random_four_quadrant(rng, 6, 6), dict(t_end=1.0, max_steps=4)))
+    cases.append(("rand65x131", problems.random_four_quadrant(rng, 65, 131), dict(t_end=1.0, max_steps=5)))
+    cases.append(("rand125x40", problems.random_four_quadrant(rng, 125, 40), dict(t_end=1.0, max_steps=5)))
+    cases.append(("2mach24", problems.initialize_2Mach(24), dict(t_end=0.01, max_steps=12, strategy=4,
+                                                                   width=4.0, height=1.0)))
+    cases.append(("reflective4cds", problems.initialize_4cds(40), dict(t_end=0.05, max_steps=10, strategy=1)))
+    cases.append(("jet30", problems.initialize_jet(30), dict(t_end=1e-3, max_steps=8, strategy=2,
+                                                               width=2.0, height=1.0)))
+    cases.append(("jet800_20", problems.initialize_jet800(20), dict(t_end=1e-3, max_steps=8, strategy=3,
+                                                                      width=3.0, height=1.0)))
+    cases.append(("rp70x130", problems.set_rp_initial_data(
+        [1.0, 1.0, 0.75, -0.5, 0], [2.0, 1.0, 0.75, 0.5, 0], [1.0, 1.0, -0.75, 0.5, 0],
+        [3.0, 1.0, -0.75, -0.5, 0], 70, 130), dict(t_end=0.02, alpha=1.7, cfl=0.4)))
+    return cases
+
+
+@pytest.fixture(scope="module")
+def cases(problems):
+    return _cases(problems)
+
+
+@pytest.mark.parametrize("idx", range(10))
+def test_exact_mode_bitwise(gpu, orc, cases, idx):
+    name, U0, prm = cases[idx]
+    print(f"case {name} shape {U0.shape} {prm}", flush=True)
+    ref, steps_ref, _, _ = orc.solve(U0, nthreads=4, **prm)
+    got, steps, _ = gpu.solve(U0, mode=gpu.MODE_EXACT, **prm)
+    assert steps == steps_ref
+    diff = np.argwhere(got[..., :4] != ref[..., :4])
+    assert diff.size == 0, f"{name}: {len(diff)} differing values, first {diff[:5].tolist()}"
+
+
+@pytest.mark.parametrize("idx", range(10))
+def test_fast_mode_parity(gpu, orc, cases, idx):
+    name, U0, prm = cases[idx]
+    print(f"case {name} shape {U0.shape} {prm}", flush=True)
+    ref, steps_ref, _, _ = orc.solve(U0, nthreads=4, **prm)
+    got, steps, _ = gpu.solve(U0, mode=gpu.MODE_FAST, **prm)
+    l1, linf = rel_errors(got, ref)
+    print(f"{name}: steps {steps}/{steps_ref} relL1 {l1} relLinf {linf}")
+    assert abs(steps - steps_ref) <= 1
+    assert np.all(l1 <= TOL_L1), (name, l1, linf)
+
+
+def test_config1_256(gpu, orc, problems):
+    """256^2 four-quadrant to t_end = 0.3 (the reference's host-runnable case)."""
+    U0 = problems.initialize_4shocks(256)
+    prm = dict(t_end=0.3)
+    ref, steps_ref, t_ref, _ = orc.solve(U0, nthreads=0, **prm)
+    ex, steps_ex, _ = gpu.solve(U0, mode=gpu.MODE_EXACT, **prm)
+    assert steps_ex == steps_ref == 403
+    assert np.array_equal(ex[..., :4], ref[..., :4])
+    fa, steps_fa, _ = gpu.solve(U0, mode=gpu.MODE_FAST, **prm)
+    l1, linf = rel_errors(fa, ref)
+    print(f"256^2 fast: steps {steps_fa} relL1 {l1} relLinf {linf}")
+    assert steps_fa == steps_ref
+    assert np.all(l1 <= TOL_L1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_launch_geometry_invariance(gpu, problems, mode):
+    """Results do not depend on strip height, waves per block or the
+    persistent ticket scheduler (bitwise)."""
+    rng = problems.SplitMix64(11)
+    U0 = problems.random_four_quadrant(rng, 133, 97)
+    base = None
+    for (R, W, P) in [(16, 4, 0), (4, 1, 1), (7, 2, 0), (64, 4, 1), (33, 1, 0)]:
+        s = gpu.Solver(133, 97, t_end=1.0, max_steps=7, mode=mode)
+        s.set_launch(R, W, P)
+        s.upload(U0)
+        steps, _, _ = s.run()
+        U = s.download()
+        s.close()
+        assert steps == 7
+        if base is None:
+            base = U
+        else:
+            assert np.array_equal(U, base), (R, W, P)
+
+
+def test_resume_like_reference(gpu, problems):
+    """A second solve() continues from the previous result, t restarts at 0
+    and the step count accumulates (src/wrappers.cuh:191,252,290-291)."""
+    U0 = problems.initialize_4shocks(48)
+    s = gpu.Solver(48, 48, t_end=0.02, mode=gpu.MODE_EXACT)
+    s.upload(U0)
+    n1, t1, _ = s.run()
+    U1 = s.download()
+    n2, t2, _ = s.run()
+    U2 = s.download()
+    one, n_one, _ = gpu.solve(U1, t_end=0.02, mode=gpu.MODE_EXACT)
+    assert s.steps_total() == n1 + n2
+    assert np.array_equal(U2, one) and n_one == n2
+    assert t1 == pytest.approx(0.02) and t2 == pytest.approx(0.02)
+
+
+def test_batch_matches_plain(gpu, problems):
+    """Each instance of a batched run equals its own plain solve (bitwise)."""
+    rng = problems.SplitMix64(5)
+    Us = [problems.random_four_quadrant(rng, nx, ny)
+          for nx, ny in [(40, 33), (128, 64), (17, 90), (200, 150), (64, 64)]]
+    steps = [6, 9, 3, 7, 5]
+    for mode in (0, 1):
+        prms = [dict(t_end=10.0, max_steps=s, mode=mode) for s in steps]
+        b = gpu.Batch([U.shape[:2] for U in Us], prms)
+        for i, U in enumerate(Us):
+            b.upload(i, U)
+        b.begin()
+        b.run(12)
+        for i, U in enumerate(Us):
+            n, _ = b.status(i)
+            assert n == steps[i]
+            got = b.download(i)
+            ref, n_ref, _ = gpu.solve(U, **prms[i])
+            assert n_ref == steps[i]
+            assert np.array_equal(got, ref), (mode, i)
+        b.close()
+
+
+def test_nan_state_reports_error(gpu, problems):
+    U0 = problems.initialize_4shocks(32)
+    U0[10, 10, 1] = -1.0  # negative pressure -> NaN sound speed
+    with pytest.raises(gpu.FVM2DError, match="non-finite"):
+        gpu.solve(U0, t_end=0.1)
