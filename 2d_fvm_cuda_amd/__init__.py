@@ -64,10 +64,12 @@ EXPORTS = [
     "fvm2d_create", "fvm2d_set_params", "fvm2d_upload", "fvm2d_upload_planes_device",
     "fvm2d_download_planes_device", "fvm2d_run", "fvm2d_download", "fvm2d_steps_total",
     "fvm2d_destroy", "fvm2d_set_launch", "fvm2d_get_launch", "fvm2d_autotune", "fvm2d_begin",
-    "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps",
+    "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
     "fvm2d_batch_autotune", "fvm2d_batch_destroy",
+    "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
+    "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
 
 _lib = None
@@ -106,6 +108,7 @@ def lib():
         "fvm2d_advance": (ctypes.c_int, [_vp, ctypes.c_int]),
         "fvm2d_sync": (ctypes.c_int, [_vp, _ip, _dp, _dp]),
         "fvm2d_time_steps": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "fvm2d_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "fvm2d_batch_create": (ctypes.c_int, [ctypes.c_int, _ip, _ip, P, ctypes.POINTER(_vp)]),
         "fvm2d_batch_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "fvm2d_batch_begin": (ctypes.c_int, [_vp]),
@@ -249,6 +252,13 @@ class Solver:
         check(lib().fvm2d_sync(self.h, ctypes.byref(s), ctypes.byref(t), ctypes.byref(dt)),
               "fvm2d_sync")
         return s.value, t.value, dt.value
+
+    def time_kernel(self, nsteps: int):
+        """(avg_ms, min_ms) of single step-kernel launches (HIP events)."""
+        a, m = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(lib().fvm2d_time_kernel(self.h, int(nsteps), ctypes.byref(a), ctypes.byref(m)),
+              "fvm2d_time_kernel")
+        return a.value, m.value
 
     def time_steps(self, nsteps: int) -> float:
         ms = ctypes.c_double(0.0)

@@ -1,0 +1,166 @@
+"""Differentiable solve: the hand-written adjoint of libfvm2d behind
+``torch.autograd.Function``.
+
+    adj = Adjoint([(ny, nx)], [params], nsteps=400, checkpoint_slots=32)
+    U_N = FVMSolve.apply(adj, U0)          # U0: cuda float64 [4, ny, nx]
+    loss = 0.5 * ((U_N[0] - target) ** 2).sum()
+    loss.backward()                        # U0.grad = dL/dU0
+
+States are conserved planes (rho, rho u, rho v, E) in float64 on the GPU,
+row 0 = north, ghost layers included.  The forward runs ``nsteps`` steps with
+dt = CFL h / s_max each (no t_end clamp) and stores binomial checkpoints; the
+backward recomputes from them and runs the reverse of every step
+(fvm2d_adj_backward).  Several instances (a batch) advance together:
+``FVMSolve.apply(adj, U0_a, U0_b, ...)`` returns one tensor per instance.
+"""
+from __future__ import annotations
+
+import ctypes
+
+_vp = ctypes.c_void_p
+
+
+class AdjStats(ctypes.Structure):
+    _fields_ = [
+        ("forward_ms", ctypes.c_double),
+        ("backward_ms", ctypes.c_double),
+        ("forward_step_ms", ctypes.c_double),
+        ("reverse_step_ms", ctypes.c_double),
+        ("reverse_steps", ctypes.c_int),
+        ("recompute_steps", ctypes.c_int),
+        ("checkpoint_slots", ctypes.c_int),
+        ("schedule_cost", ctypes.c_int),
+    ]
+
+
+EXPORTS = ["fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
+           "fvm2d_adj_set_launch", "fvm2d_adj_destroy"]
+
+
+def bind(L):
+    from . import Params
+    P = ctypes.POINTER(Params)
+    ip = ctypes.POINTER(ctypes.c_int)
+    pp = ctypes.POINTER(_vp)
+    L.fvm2d_adj_create.argtypes = [ctypes.c_int, ip, ip, P, ctypes.c_int, ctypes.c_int, pp]
+    L.fvm2d_adj_create.restype = ctypes.c_int
+    L.fvm2d_adj_forward.argtypes = [_vp, pp, pp]
+    L.fvm2d_adj_forward.restype = ctypes.c_int
+    L.fvm2d_adj_backward.argtypes = [_vp, pp, pp]
+    L.fvm2d_adj_backward.restype = ctypes.c_int
+    L.fvm2d_adj_get_stats.argtypes = [_vp, ctypes.POINTER(AdjStats)]
+    L.fvm2d_adj_get_stats.restype = ctypes.c_int
+    L.fvm2d_adj_set_launch.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.fvm2d_adj_set_launch.restype = ctypes.c_int
+    L.fvm2d_adj_destroy.argtypes = [_vp]
+    L.fvm2d_adj_destroy.restype = ctypes.c_int
+
+
+def _ptrs(tensors):
+    arr = (_vp * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+class Adjoint:
+    """A forward/adjoint pair for one or more meshes (fvm2d_adj_*)."""
+
+    def __init__(self, shapes, params_list, nsteps: int, checkpoint_slots: int = 0):
+        from . import check, lib, make_params, Params
+        self._lib = lib()
+        n = len(shapes)
+        self.n = n
+        self.shapes = [(int(a), int(b)) for a, b in shapes]
+        self.nsteps = int(nsteps)
+        nx = (ctypes.c_int * n)(*[s[1] for s in self.shapes])
+        ny = (ctypes.c_int * n)(*[s[0] for s in self.shapes])
+        P = (Params * n)(*[make_params(**pp) for pp in params_list])
+        h = _vp()
+        check(self._lib.fvm2d_adj_create(n, nx, ny, P, self.nsteps, int(checkpoint_slots),
+                                         ctypes.byref(h)), "fvm2d_adj_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fvm2d_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_states(self, ts):
+        import torch
+        assert len(ts) == self.n
+        for t, (ny, nx) in zip(ts, self.shapes):
+            if t.dtype != torch.float64 or not t.is_cuda or tuple(t.shape) != (4, ny, nx):
+                raise ValueError(f"expected a cuda float64 tensor of shape (4, {ny}, {nx})")
+            if not t.is_contiguous():
+                raise ValueError("state tensors must be contiguous")
+
+    def forward(self, U0s):
+        import torch
+        from . import check
+        self._check_states(U0s)
+        outs = [torch.empty_like(u) for u in U0s]
+        torch.cuda.synchronize()
+        check(self._lib.fvm2d_adj_forward(self.h, _ptrs(U0s), _ptrs(outs)), "fvm2d_adj_forward")
+        return outs
+
+    def backward(self, gNs):
+        import torch
+        from . import check
+        gNs = [g.contiguous() for g in gNs]
+        self._check_states(gNs)
+        outs = [torch.empty_like(g) for g in gNs]
+        torch.cuda.synchronize()
+        check(self._lib.fvm2d_adj_backward(self.h, _ptrs(gNs), _ptrs(outs)), "fvm2d_adj_backward")
+        return outs
+
+    def stats(self) -> dict:
+        from . import check
+        st = AdjStats()
+        check(self._lib.fvm2d_adj_get_stats(self.h, ctypes.byref(st)), "fvm2d_adj_get_stats")
+        return {k: getattr(st, k) for k, _ in AdjStats._fields_}
+
+    def set_launch(self, strip_rows=0, waves_per_block=0, persistent=-1):
+        from . import check
+        check(self._lib.fvm2d_adj_set_launch(self.h, strip_rows, waves_per_block, persistent),
+              "fvm2d_adj_set_launch")
+
+
+def _function():
+    import torch
+
+    class FVMSolve(torch.autograd.Function):
+        """U_N = solve(U_0) for every instance of ``adj``; backward is the
+        hand-written adjoint with binomial checkpointing."""
+
+        @staticmethod
+        def forward(ctx, adj, *U0s):
+            ctx.adj = adj
+            outs = adj.forward([u.detach().contiguous() for u in U0s])
+            return outs[0] if len(outs) == 1 else tuple(outs)
+
+        @staticmethod
+        def backward(ctx, *gNs):
+            gs = ctx.adj.backward(list(gNs))
+            return (None, *gs)
+
+    return FVMSolve
+
+
+_cache = {}
+
+
+def __getattr__(name):
+    """``FVMSolve`` is created on first access so that importing this module
+    (which the library loader does) does not import torch."""
+    if name == "FVMSolve":
+        if name not in _cache:
+            _cache[name] = _function()
+        return _cache[name]
+    raise AttributeError(name)

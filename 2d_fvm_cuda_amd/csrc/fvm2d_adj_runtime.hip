@@ -1,0 +1,480 @@
+// Adjoint front end of include/fvm2d.h: forward solve with binomial
+// time-step checkpointing, then the reverse sweep of every step.
+//
+// Checkpoint schedule.  cost[n][c] is the least number of forward steps
+// needed to reverse n steps whose first state is stored, with c further
+// checkpoint slots free:
+//     cost[1][c] = 0,  cost[n][0] = n (n - 1) / 2,
+//     cost[n][c] = min_{0<m<n} m + cost[n-m][c-1] + cost[m][c],
+// the recurrence whose minimum is the binomial (revolve) bound; split[n][c]
+// keeps the minimising m.  The forward call stores U_0 and the checkpoints
+// on the schedule's first spine; the backward call follows the recursion,
+// recomputing from the nearest stored state.
+#include <map>
+
+#include "fvm2d_adjoint_kernels.h"
+#include "fvm2d_engine.h"
+
+namespace fvm2d {
+
+struct AdjEngine {
+    Engine fwd;                      // forward stepping of all instances
+    int n = 0, nsteps = 0, slots = 0;
+    std::vector<int> nx, ny;
+    std::vector<double> gamma;
+    // per instance device buffers
+    std::vector<double *> ck;        // [inst * slots + s] checkpoint states
+    std::vector<double *> A1, B1, lamA1, lamB1, lamX, lamY, UN, U0c;
+    std::vector<unsigned long long *> pmax;
+    std::vector<long long *> pidx;
+    double *dtpart = nullptr;
+    AdjDesc *d_adesc = nullptr;
+    AdjCtl *d_actl = nullptr;
+    std::vector<AdjDesc> had;
+    int4 *d_tfwd = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
+    int nfwd = 0, nr2 = 0, nr3a = 0, nr3b = 0;
+    std::vector<void *> allocs;
+    // schedule
+    std::vector<std::vector<long long>> cost;
+    std::vector<std::vector<int>> split;
+    std::vector<std::vector<double>> slot_t;  // [slot][inst] time of the stored state
+    std::vector<int> slot_step;      // step index held by each slot (-1: free)
+    std::map<int, int> stored;       // step -> slot
+    // stats
+    long long recompute_steps = 0;
+    int reverse_steps = 0;
+    double fwd_ms = 0.0, bwd_ms = 0.0, rev_ms_sum = 0.0;
+    hipEvent_t r0 = nullptr, r1 = nullptr;
+    bool have_forward = false;
+
+    ~AdjEngine() {
+        for (void *p : allocs) hipFree(p);
+        if (r0) hipEventDestroy(r0);
+        if (r1) hipEventDestroy(r1);
+    }
+
+    template <class T>
+    int dalloc(T **p, size_t count) {
+        HIPCHK(hipMalloc((void **)p, std::max<size_t>(1, count) * sizeof(T)));
+        allocs.push_back((void *)*p);
+        return FVM2D_OK;
+    }
+
+    int init(int count, const int *nx_, const int *ny_, const fvm2d_params *p, int steps, int nslots) {
+        if (count < 1 || steps < 1) return fail(FVM2D_ERR_ARG, "need at least one instance and one step");
+        n = count;
+        nsteps = steps;
+        std::vector<fvm2d_params> fp(p, p + count);
+        for (auto &q : fp) {
+            q.t_end = std::numeric_limits<double>::infinity();
+            q.max_steps = -1;
+            q.mode = FVM2D_MODE_FAST;
+        }
+        int rc = fwd.init(count, nx_, ny_, fp.data());
+        if (rc) return rc;
+        slots = std::max(1, std::min(nslots <= 0 ? steps : nslots, steps));
+        nx.assign(nx_, nx_ + count);
+        ny.assign(ny_, ny_ + count);
+        HIPCHK(hipEventCreate(&r0));
+        HIPCHK(hipEventCreate(&r1));
+        had.resize(n);
+        ck.resize((size_t)n * slots);
+        A1.resize(n), B1.resize(n), lamA1.resize(n), lamB1.resize(n), lamX.resize(n), lamY.resize(n);
+        UN.resize(n), U0c.resize(n), pmax.resize(n), pidx.resize(n);
+        rc = dalloc(&d_actl, n);
+        if (rc) return rc;
+        for (int i = 0; i < n; i++) {
+            const InstDesc &d = fwd.hd[i];
+            const size_t st = 4 * (size_t)d.plane;
+            for (int s = 0; s < slots; s++)
+                if ((rc = dalloc(&ck[(size_t)i * slots + s], st))) return rc;
+            for (double **b : {&A1[i], &B1[i], &lamA1[i], &lamB1[i], &lamX[i], &lamY[i], &UN[i]})
+                if ((rc = dalloc(b, st))) return rc;
+            if ((rc = dalloc(&U0c[i], 4 * (size_t)nx[i] * ny[i]))) return rc;
+            const size_t npairs = ((size_t)nx[i] * ny[i] + 63) / 64;
+            if ((rc = dalloc(&pmax[i], npairs))) return rc;
+            if ((rc = dalloc(&pidx[i], npairs))) return rc;
+            HIPCHK(hipMemsetAsync(A1[i], 0, st * sizeof(double), fwd.stream));
+            HIPCHK(hipMemsetAsync(B1[i], 0, st * sizeof(double), fwd.stream));
+            AdjDesc &a = had[i];
+            a.nx = d.nx;
+            a.ny = d.ny;
+            a.pitch = d.pitch;
+            a.strategy = d.strategy;
+            a.plane = d.plane;
+            a.cfl = d.cfl;
+            a.hmin = d.hmin;
+            a.lx = d.lx;
+            a.ly = d.ly;
+            a.gas = d.gas;
+            a.A1 = A1[i];
+            a.B1 = B1[i];
+            a.lamA1 = lamA1[i];
+            a.lamB1 = lamB1[i];
+            a.pair_max = pmax[i];
+            a.pair_idx = pidx[i];
+            a.ctl = d_actl + i;
+            gamma.push_back(p[i].gamma);
+        }
+        rc = build_tasks();
+        if (rc) return rc;
+        rc = dalloc(&d_adesc, n);
+        if (rc) return rc;
+        build_schedule();
+        HIPCHK(hipStreamSynchronize(fwd.stream));
+        return FVM2D_OK;
+    }
+
+    int build_tasks() {
+        std::vector<int4> tf, t2, t3a, t3b;
+        for (int i = 0; i < n; i++) {
+            const int wf_x = (nx[i] + 59) / 60, wf_y = (ny[i] + 59) / 60;
+            const int wa_x = (nx[i] + 55) / 56, wa_y = (ny[i] + 55) / 56;
+            for (int j = 0; j < ny[i]; j++)
+                for (int w = 0; w < wf_x; w++) tf.push_back(make_int4(i, 0, w, j));
+            for (int c = 0; c < nx[i]; c++)
+                for (int w = 0; w < wf_y; w++) tf.push_back(make_int4(i, 1, w, c));
+        }
+        int base = 0;
+        for (int i = 0; i < n; i++) {  // R2: X^T on B1 (rows) and Y^T on A1 (columns)
+            had[i].part_lo[0] = base + (int)t2.size();
+            const int wa_x = (nx[i] + 55) / 56, wa_y = (ny[i] + 55) / 56;
+            for (int j = 0; j < ny[i]; j++)
+                for (int w = 0; w < wa_x; w++) t2.push_back(make_int4(i, 0, w, j));
+            for (int c = 0; c < nx[i]; c++)
+                for (int w = 0; w < wa_y; w++) t2.push_back(make_int4(i, 1, w, c));
+            had[i].part_hi[0] = base + (int)t2.size();
+        }
+        base += (int)t2.size();
+        const int base3a = base;
+        for (int i = 0; i < n; i++) {  // R3a: X^T on U (rows)
+            had[i].part_lo[1] = base3a + (int)t3a.size();
+            const int wa_x = (nx[i] + 55) / 56;
+            for (int j = 0; j < ny[i]; j++)
+                for (int w = 0; w < wa_x; w++) t3a.push_back(make_int4(i, 0, w, j));
+            had[i].part_hi[1] = base3a + (int)t3a.size();
+        }
+        base += (int)t3a.size();
+        const int base3b = base;
+        for (int i = 0; i < n; i++) {  // R3b: Y^T on U (columns)
+            had[i].part_lo[2] = base3b + (int)t3b.size();
+            const int wa_y = (ny[i] + 55) / 56;
+            for (int c = 0; c < nx[i]; c++)
+                for (int w = 0; w < wa_y; w++) t3b.push_back(make_int4(i, 1, w, c));
+            had[i].part_hi[2] = base3b + (int)t3b.size();
+        }
+        base += (int)t3b.size();
+        nfwd = (int)tf.size();
+        nr2 = (int)t2.size();
+        nr3a = (int)t3a.size();
+        nr3b = (int)t3b.size();
+        int rc;
+        if ((rc = dalloc(&dtpart, base))) return rc;
+        if ((rc = dalloc(&d_tfwd, nfwd))) return rc;
+        if ((rc = dalloc(&d_tr2, nr2))) return rc;
+        if ((rc = dalloc(&d_tr3a, nr3a))) return rc;
+        if ((rc = dalloc(&d_tr3b, nr3b))) return rc;
+        for (int i = 0; i < n; i++) had[i].dtpart = dtpart;
+        HIPCHK(hipMemcpyAsync(d_tfwd, tf.data(), nfwd * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
+        HIPCHK(hipMemcpyAsync(d_tr2, t2.data(), nr2 * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
+        HIPCHK(hipMemcpyAsync(d_tr3a, t3a.data(), nr3a * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
+        HIPCHK(hipMemcpyAsync(d_tr3b, t3b.data(), nr3b * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
+        HIPCHK(hipStreamSynchronize(fwd.stream));
+        return FVM2D_OK;
+    }
+
+    void build_schedule() {
+        const int N = nsteps, C = slots - 1;  // slot 0 always holds U_0
+        cost.assign(N + 1, std::vector<long long>(C + 1, 0));
+        split.assign(N + 1, std::vector<int>(C + 1, 0));
+        for (int m = 1; m <= N; m++) cost[m][0] = (long long)m * (m - 1) / 2;
+        for (int c = 1; c <= C; c++) {
+            cost[1][c] = 0;
+            for (int m = 2; m <= N; m++) {
+                long long best = -1;
+                int bs = 1;
+                for (int s = 1; s < m; s++) {
+                    const long long v = s + cost[m - s][c - 1] + cost[s][c];
+                    if (best < 0 || v < best) {
+                        best = v;
+                        bs = s;
+                    }
+                }
+                cost[m][c] = best;
+                split[m][c] = bs;
+            }
+        }
+    }
+
+    // Copy the state of every instance from slot src (or the forward
+    // engine's buf0 when src < 0) to slot dst (or buf0 when dst < 0).
+    int copy_state(int src, int dst) {
+        for (int i = 0; i < n; i++) {
+            const InstDesc &d = fwd.hd[i];
+            const double *a = src < 0 ? d.buf0 : ck[(size_t)i * slots + src];
+            double *b = dst < 0 ? d.buf0 : ck[(size_t)i * slots + dst];
+            HIPCHK(hipMemcpyAsync(b, a, 4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice, fwd.stream));
+        }
+        return FVM2D_OK;
+    }
+
+    // Advance every instance by m steps from the state in slot src (times
+    // t_src per instance); the result is left in the forward engine's buf0
+    // and its times in t_out.
+    int advance(int src, int m, const std::vector<double> &t_src, std::vector<double> *t_out) {
+        int rc = copy_state(src, -1);
+        if (rc) return rc;
+        rc = fwd.begin(t_src.data());
+        if (rc) return rc;
+        if (m > 0) {
+            rc = fwd.enqueue(m);
+            if (rc) return rc;
+        }
+        rc = fwd.read_ctl();
+        if (rc) return rc;
+        for (int i = 0; i < n; i++) {
+            if (fwd.h_ctl[i].status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite state in the forward run");
+            if (m & 1) std::swap(fwd.hd[i].buf0, fwd.hd[i].buf1);
+        }
+        if (m & 1) {
+            rc = fwd.upload_desc();
+            if (rc) return rc;
+        }
+        recompute_steps += m;
+        if (t_out) {
+            t_out->resize(n);
+            for (int i = 0; i < n; i++) (*t_out)[i] = fwd.h_ctl[i].t;
+        }
+        return FVM2D_OK;
+    }
+
+    int take_slot() const {
+        for (int s = 1; s < slots; s++)
+            if (slot_step[s] < 0) return s;
+        return -1;
+    }
+
+    // Reverse step n (U_n in slot s, or in buf0 when s < 0): lamX (dL/dU_{n+1})
+    // -> lamY (dL/dU_n); then the two are swapped.
+    int reverse_step(int s) {
+        HIPCHK(hipEventRecord(r0, fwd.stream));
+        for (int i = 0; i < n; i++) {
+            had[i].U = s < 0 ? fwd.hd[i].buf0 : ck[(size_t)i * slots + s];
+            had[i].lam_in = lamX[i];
+            had[i].lam_out = lamY[i];
+        }
+        HIPCHK(hipMemcpyAsync(d_adesc, had.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, fwd.stream));
+        for (int i = 0; i < n; i++) adj_launch_smax(d_adesc, i, (long long)nx[i] * ny[i], fwd.stream);
+        adj_launch_dt(d_adesc, n, fwd.stream);
+        adj_launch_fwd(d_adesc, d_tfwd, nfwd, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr2, nr2, 0, 0, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr3a, nr3a, 1, nr2, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr3b, nr3b, 2, nr2 + nr3a, fwd.stream);
+        adj_launch_dt_term(d_adesc, n, nr2 + nr3a + nr3b, fwd.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(r1, fwd.stream));
+        HIPCHK(hipEventSynchronize(r1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r0, r1));
+        rev_ms_sum += ms;
+        reverse_steps++;
+        std::swap(lamX, lamY);
+        return FVM2D_OK;
+    }
+
+    // Reverse steps a .. a+len-1; the state U_a is in slot sa.
+    int reverse(int a, int len, int c, int sa) {
+        if (len == 1) return reverse_step(sa);
+        if (c == 0 || take_slot() < 0) {
+            for (int k = len - 1; k >= 1; k--) {
+                int rc = advance(sa, k, slot_t[sa], nullptr);
+                if (rc) return rc;
+                rc = reverse_step(-1);
+                if (rc) return rc;
+            }
+            return reverse_step(sa);
+        }
+        const int m = split[len][std::min(c, (int)split[len].size() - 1)];
+        int s;
+        auto it = stored.find(a + m);
+        if (it != stored.end()) {
+            s = it->second;
+        } else {
+            s = take_slot();
+            std::vector<double> t;
+            int rc = advance(sa, m, slot_t[sa], &t);
+            if (rc) return rc;
+            rc = copy_state(-1, s);
+            if (rc) return rc;
+            slot_t[s] = t;
+            slot_step[s] = a + m;
+            stored[a + m] = s;
+        }
+        int rc = reverse(a + m, len - m, c - 1, s);
+        if (rc) return rc;
+        stored.erase(a + m);
+        slot_step[s] = -1;
+        return reverse(a, m, c, sa);
+    }
+
+    int forward(const double *const *U0_cons, double *const *UN_cons) {
+        for (int i = 0; i < n; i++) {
+            if (!U0_cons[i] || !UN_cons[i]) return fail(FVM2D_ERR_ARG, "null state pointer");
+            const InstDesc &d = fwd.hd[i];
+            HIPCHK(hipMemcpyAsync(U0c[i], U0_cons[i], 4 * (size_t)nx[i] * ny[i] * sizeof(double),
+                                  hipMemcpyDeviceToDevice, fwd.stream));
+            adj_launch_cons_to_prim(U0c[i], ck[(size_t)i * slots], nx[i], ny[i], d.pitch, d.plane,
+                                    gamma[i] - 1.0, fwd.stream);
+        }
+        HIPCHK(hipGetLastError());
+        slot_t.assign(slots, std::vector<double>(n, 0.0));
+        slot_step.assign(slots, -1);
+        stored.clear();
+        slot_step[0] = 0;
+        stored[0] = 0;
+        recompute_steps = 0;
+        HIPCHK(hipEventRecord(r0, fwd.stream));
+        // walk the first spine of the schedule, storing its checkpoints
+        int a = 0, len = nsteps, c = slots - 1, sa = 0;
+        std::vector<double> t;
+        while (len > 1 && c > 0) {
+            const int m = split[len][c];
+            const int s = take_slot();
+            int rc = advance(sa, m, slot_t[sa], &t);
+            if (rc) return rc;
+            rc = copy_state(-1, s);
+            if (rc) return rc;
+            slot_t[s] = t;
+            slot_step[s] = a + m;
+            stored[a + m] = s;
+            a += m;
+            len -= m;
+            c -= 1;
+            sa = s;
+        }
+        int rc = advance(sa, len, slot_t[sa], &t);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(r1, fwd.stream));
+        HIPCHK(hipEventSynchronize(r1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r0, r1));
+        fwd_ms = ms;
+        for (int i = 0; i < n; i++) {
+            const InstDesc &d = fwd.hd[i];
+            HIPCHK(hipMemcpyAsync(UN[i], d.buf0, 4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice,
+                                  fwd.stream));
+            adj_launch_prim_to_cons(d.buf0, UN_cons[i], nx[i], ny[i], d.pitch, d.plane, 1.0 / (gamma[i] - 1.0),
+                                    fwd.stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(fwd.stream));
+        forward_recompute_base = recompute_steps;
+        have_forward = true;
+        return FVM2D_OK;
+    }
+    long long forward_recompute_base = 0;
+
+    int backward(const double *const *gN_cons, double *const *g0_cons) {
+        if (!have_forward) return fail(FVM2D_ERR_STATE, "backward called before forward");
+        for (int i = 0; i < n; i++) {
+            if (!gN_cons[i] || !g0_cons[i]) return fail(FVM2D_ERR_ARG, "null gradient pointer");
+            const InstDesc &d = fwd.hd[i];
+            HIPCHK(hipMemsetAsync(lamX[i], 0, 4 * d.plane * sizeof(double), fwd.stream));
+            adj_launch_cons_grad_to_prim(UN[i], gN_cons[i], lamX[i], nx[i], ny[i], d.pitch, d.plane,
+                                         1.0 / (gamma[i] - 1.0), fwd.stream);
+        }
+        HIPCHK(hipGetLastError());
+        recompute_steps = 0;
+        reverse_steps = 0;
+        rev_ms_sum = 0.0;
+        hipEvent_t b0, b1;
+        HIPCHK(hipEventCreate(&b0));
+        HIPCHK(hipEventCreate(&b1));
+        HIPCHK(hipEventRecord(b0, fwd.stream));
+        int rc = reverse(0, nsteps, slots - 1, 0);
+        if (rc) {
+            hipEventDestroy(b0);
+            hipEventDestroy(b1);
+            return rc;
+        }
+        for (int i = 0; i < n; i++) {
+            const InstDesc &d = fwd.hd[i];
+            adj_launch_prim_grad_to_cons(U0c[i], lamX[i], g0_cons[i], nx[i], ny[i], d.pitch, d.plane,
+                                         gamma[i] - 1.0, fwd.stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b1, fwd.stream));
+        HIPCHK(hipEventSynchronize(b1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b0, b1));
+        bwd_ms = ms;
+        hipEventDestroy(b0);
+        hipEventDestroy(b1);
+        // every forward state used by the schedule after the forward call
+        std::vector<AdjCtl> ac(n);
+        HIPCHK(hipMemcpy(ac.data(), d_actl, n * sizeof(AdjCtl), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++)
+            if (ac[i].status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite wave speed in the reverse sweep");
+        return FVM2D_OK;
+    }
+};
+
+}  // namespace fvm2d
+
+using namespace fvm2d;
+
+struct fvm2d_adj {
+    AdjEngine eng;
+};
+
+extern "C" {
+
+int fvm2d_adj_create(int n, const int *nx, const int *ny, const fvm2d_params *p, int nsteps,
+                     int checkpoint_slots, fvm2d_adj **out) {
+    if (!out || !nx || !ny || !p) return fail(FVM2D_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto *a = new fvm2d_adj();
+    int rc = a->eng.init(n, nx, ny, p, nsteps, checkpoint_slots);
+    if (rc) {
+        delete a;
+        return rc;
+    }
+    *out = a;
+    return FVM2D_OK;
+}
+
+int fvm2d_adj_forward(fvm2d_adj *a, const double *const *d_U0_cons, double *const *d_UN_cons) {
+    if (!a || !d_U0_cons || !d_UN_cons) return fail(FVM2D_ERR_ARG, "null argument");
+    return a->eng.forward(d_U0_cons, d_UN_cons);
+}
+
+int fvm2d_adj_backward(fvm2d_adj *a, const double *const *d_gN_cons, double *const *d_g0_cons) {
+    if (!a || !d_gN_cons || !d_g0_cons) return fail(FVM2D_ERR_ARG, "null argument");
+    return a->eng.backward(d_gN_cons, d_g0_cons);
+}
+
+int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *st) {
+    if (!a || !st) return fail(FVM2D_ERR_ARG, "null argument");
+    const AdjEngine &E = a->eng;
+    st->forward_ms = E.fwd_ms;
+    st->backward_ms = E.bwd_ms;
+    st->reverse_step_ms = E.reverse_steps ? E.rev_ms_sum / E.reverse_steps : 0.0;
+    st->forward_step_ms = E.nsteps ? E.fwd_ms / E.nsteps : 0.0;
+    st->reverse_steps = E.reverse_steps;
+    st->recompute_steps = (int)E.recompute_steps;
+    st->checkpoint_slots = E.slots;
+    st->schedule_cost = (int)E.cost[E.nsteps][E.slots - 1];
+    return FVM2D_OK;
+}
+
+int fvm2d_adj_set_launch(fvm2d_adj *a, int strip_rows, int waves_per_block, int persistent) {
+    if (!a) return fail(FVM2D_ERR_ARG, "adjoint is null");
+    return a->eng.fwd.set_launch(strip_rows, waves_per_block, persistent);
+}
+
+int fvm2d_adj_destroy(fvm2d_adj *a) {
+    delete a;
+    return FVM2D_OK;
+}
+
+}  // extern "C"

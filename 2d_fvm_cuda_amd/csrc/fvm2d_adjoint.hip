@@ -1,0 +1,452 @@
+// Reverse (adjoint) step of the split scheme on gfx950.
+//
+// Forward step (fixed by the state U_n):  dt = CFL h / s_max(U_n)
+//     A1 = X(U), A = Y(A1);  B1 = Y(U), B = X(B1);  U' = fill((A + B)/2)
+// Reverse step, given lam' = dL/dU' (primitive planes, every cell):
+//   R1  recompute A1 = X(U), B1 = Y(U); s_max(U) with its arg-max cell, dt
+//   R2  lam_A1 = Y^T(A1; lam_C/2),  lam_B1 = X^T(B1; lam_C/2)
+//       where lam_C = fill^T(lam') is gathered while loading
+//   R3  lam = X^T(U; lam_A1) + Y^T(U; lam_B1)
+//   R4  lam[argmax] += dtbar * d dt / d U   (dt depends on the arg-max cell)
+// Every sweep adjoint is one lane-parallel routine: lanes are 64
+// consecutive cells of one line; an X line is a row (unit stride), a Y line
+// is a column read with the row pitch as stride, in the orientation of the
+// reference's rotated sweep (line index k = ny-1-j) and with u and v
+// swapped.  The routine recomputes the sweep's forward intermediates (limited
+// slopes, faces, GRP states, fluxes, updated states) and applies the
+// hand-written adjoints of fvm2d_adjoint_cell.h; a cell's adjoint depends on
+// cells up to four away, so 56 of the 64 lanes produce output.
+#include <hip/hip_runtime.h>
+
+#include "fvm2d_adjoint_cell.h"
+#include "fvm2d_adjoint_kernels.h"
+#include "fvm2d_ghost.h"
+#include "fvm2d_lanes.h"
+
+namespace fvm2d {
+
+constexpr int kAdjHalo = 4;
+constexpr int kAdjOwn = kWave - 2 * kAdjHalo;  // 56
+constexpr int kFwdHalo = 2;
+constexpr int kFwdOwn = kWave - 2 * kFwdHalo;  // 60
+
+// Geometry of one lane line of a task.
+struct LineMap {
+    long long base;    // offset of line index 0
+    long long stride;  // offset step per line index (+1 for X, -pitch for Y)
+    int n;             // cells on the line
+    int k;             // this lane's line index
+    int kc;            // clamped into [0, n)
+    bool ydir;
+};
+
+__device__ __forceinline__ LineMap line_map(const AdjDesc &d, const int4 &t, int lane, int halo,
+                                            int own) {
+    LineMap m;
+    m.ydir = t.y != 0;
+    m.k = t.z * own - halo + lane;
+    if (!m.ydir) {  // row t.w, line index = column
+        m.n = d.nx;
+        m.base = (long long)t.w * d.pitch;
+        m.stride = 1;
+    } else {  // column t.w, line index k = ny-1-row
+        m.n = d.ny;
+        m.base = (long long)(d.ny - 1) * d.pitch + t.w;
+        m.stride = -(long long)d.pitch;
+    }
+    m.kc = m.k < 0 ? 0 : (m.k >= m.n ? m.n - 1 : m.k);
+    return m;
+}
+
+__device__ __forceinline__ long long off_of(const LineMap &m) { return m.base + m.kc * m.stride; }
+
+__device__ __forceinline__ Q load_line(const double *a, long long plane, const LineMap &m) {
+    const long long o = off_of(m);
+    const Q q{a[o], a[plane + o], a[2 * plane + o], a[3 * plane + o]};
+    return m.ydir ? swap_uv(q) : q;
+}
+
+__device__ __forceinline__ void store_line(double *a, long long plane, const LineMap &m, const Q &q,
+                                           bool accumulate) {
+    const long long o = off_of(m);
+    const Q x = m.ydir ? swap_uv(q) : q;
+    if (accumulate) {
+        a[o] += x.r;
+        a[plane + o] += x.p;
+        a[2 * plane + o] += x.un;
+        a[3 * plane + o] += x.ut;
+    } else {
+        a[o] = x.r;
+        a[plane + o] = x.p;
+        a[2 * plane + o] = x.un;
+        a[3 * plane + o] = x.ut;
+    }
+}
+
+__device__ __forceinline__ const Line &line_of(const AdjDesc &d, const LineMap &m) {
+    return m.ydir ? d.ly : d.lx;
+}
+
+__device__ __forceinline__ unsigned long long bits_of(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
+// fill^T: adjoint of the filled state U' at interior cell (j, c), plus the
+// adjoints of every ghost cell whose fill copies (j, c); ghost cells of the
+// averaged state receive nothing (the fill overwrites them).
+__device__ Q lamC_at(const AdjDesc &d, const double *lam, int j, int c) {
+    const int nx = d.nx, ny = d.ny;
+    if (j < 2 || j > ny - 3 || c < 2 || c > nx - 3) return Q{0.0, 0.0, 0.0, 0.0};
+    const long long P = d.plane;
+    long long o = (long long)j * d.pitch + c;
+    Q s{lam[o], lam[P + o], lam[2 * P + o], lam[3 * P + o]};
+    if (j == 2 || j == ny - 4 || j == ny - 3 || c == 2 || c == nx - 3) {
+        const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
+        const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
+        for (int a = 0; a < 5; a++) {
+            for (int b = 0; b < 5; b++) {
+                if (a == 0 && b == 0) continue;
+                const int gj = rows[a], gi = cols[b];
+                if (!(gj < 2 || gj > ny - 3 || gi < 2 || gi > nx - 3)) continue;
+                const GhostSrc g = ghost_source(d.strategy, gj, gi, nx, ny, 0.0);
+                if (g.is_const || g.sj != j || g.si != c) continue;
+                o = (long long)gj * d.pitch + gi;
+                s.r += lam[o];
+                s.p += lam[P + o];
+                s.un += lam[2 * P + o];
+                s.ut += g.negv ? -lam[3 * P + o] : lam[3 * P + o];
+            }
+        }
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- R1
+// Forward first sweeps A1 = X(U) (X tasks) and B1 = Y(U) (Y tasks); X tasks
+// also reduce (s_max, first arg-max index) of their owned cells.
+__global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ descs,
+                                                      const int4 *__restrict__ tasks, int ntasks) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int tk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (tk >= ntasks) return;
+    const int4 t = tasks[tk];
+    AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
+    const LineMap m = line_map(d, t, lane, kFwdHalo, kFwdOwn);
+    const Line &L = line_of(d, m);
+    const double dt = d.ctl->dt;
+    const Q q = load_line(d.U, d.plane, m);
+    const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
+    const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
+    const bool upd = m.k >= 2 && m.k <= m.n - 3;
+    if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, upd ? qo : q, false);
+}
+
+// s_max over every cell of U_n with the smallest linear index attaining it;
+// the last block turns it into dt.  One thread per cell.
+__global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ descs, int inst) {
+    AdjDesc &d = descs[inst];
+    const long long n = (long long)d.nx * d.ny;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long mx = 0ull;
+    long long idx = 0x7fffffffffffffffll;
+    if (k < n) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        const double sp = speed<false>(d.U[o], d.U[d.plane + o], d.U[2 * d.plane + o],
+                                       d.U[3 * d.plane + o], d.gas);
+        mx = bits_of(sp);
+        idx = k;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long m2 = __shfl_xor(mx, o, kWave);
+        const long long i2 = __shfl_xor(idx, o, kWave);
+        if (m2 > mx || (m2 == mx && i2 < idx)) {
+            mx = m2;
+            idx = i2;
+        }
+    }
+    // lane 0 holds the wave's first cell; waves wholly past the mesh own no
+    // pair slot (there are ceil(n / 64) slots)
+    if ((threadIdx.x & (kWave - 1)) == 0 && k < n) {
+        const long long slot = k >> 6;
+        d.pair_max[slot] = mx;
+        d.pair_idx[slot] = idx;
+    }
+}
+
+// Reduce the per-wave pairs (fixed order), set s_max, arg-max and dt.
+__global__ void __launch_bounds__(256) adj_dt_kernel(AdjDesc *__restrict__ descs, int ninst) {
+    __shared__ unsigned long long smx[256];
+    __shared__ long long sidx[256];
+    for (int inst = 0; inst < ninst; inst++) {
+        AdjDesc &d = descs[inst];
+        const int npairs = (int)(((long long)d.nx * d.ny + kWave - 1) / kWave);
+        unsigned long long mx = 0ull;
+        long long idx = 0x7fffffffffffffffll;
+        for (int p = threadIdx.x; p < npairs; p += blockDim.x) {
+            const unsigned long long m2 = d.pair_max[p];
+            const long long i2 = d.pair_idx[p];
+            if (m2 > mx || (m2 == mx && i2 < idx)) {
+                mx = m2;
+                idx = i2;
+            }
+        }
+        smx[threadIdx.x] = mx;
+        sidx[threadIdx.x] = idx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int p = 1; p < (int)blockDim.x; p++)
+                if (smx[p] > mx || (smx[p] == mx && sidx[p] < idx)) {
+                    mx = smx[p];
+                    idx = sidx[p];
+                }
+            const double smax = __longlong_as_double((long long)mx);
+            d.ctl->smax = smax;
+            d.ctl->argmax = idx;
+            d.ctl->dt = d.cfl * d.hmin / smax;
+            d.ctl->status |= (mx > 0x7ff0000000000000ull) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- R2 / R3
+// One sweep adjoint per task.  mode 0 (R2): input state A1 (Y tasks) or B1
+// (X tasks), output adjoint lam_C/2 gathered from lam_in, result written to
+// lam_A1 / lam_B1.  mode 1 (R3a, X tasks): input U, output adjoint lam_A1,
+// result written to lam_out.  mode 2 (R3b, Y tasks): input U, output adjoint
+// lam_B1, result added to lam_out.
+__global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ descs,
+                                                        const int4 *__restrict__ tasks, int ntasks,
+                                                        int mode, int part_base) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int tk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (tk >= ntasks) return;
+    const int4 t = tasks[tk];
+    AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
+    const LineMap m = line_map(d, t, lane, kAdjHalo, kAdjOwn);
+    const Line &L = line_of(d, m);
+    const Gas &G = d.gas;
+    const double dt = d.ctl->dt;
+    const double lam = dt / L.h;
+    const long long P = d.plane;
+
+    const double *qin;
+    Q qbo;  // adjoint of this sweep's output at this lane's cell
+    if (mode == 0) {
+        qin = m.ydir ? d.A1 : d.B1;
+        const long long o = off_of(m);
+        const int j = (int)(o / d.pitch), c = (int)(o % d.pitch);
+        const Q s = lamC_at(d, d.lam_in, j, c);
+        const Q h{0.5 * s.r, 0.5 * s.p, 0.5 * s.un, 0.5 * s.ut};
+        qbo = m.ydir ? swap_uv(h) : h;
+    } else {
+        qin = d.U;
+        qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
+    }
+    const Q q = load_line(qin, P, m);
+
+    const bool inl = m.k >= 0 && m.k < m.n;
+    const bool upd = inl && m.k >= 2 && m.k <= m.n - 3;
+    const bool sig = inl && m.k >= 1 && m.k <= m.n - 2;
+    const bool ifc = inl && m.k >= 2 && m.k <= m.n - 2;
+    const bool own = inl && lane >= kAdjHalo && lane < kWave - kAdjHalo;
+
+    // ---- forward recompute of the sweep
+    const Q qL = shr(q), qR = shl(q);
+    const Q s = slope<false>(qL, q, qR, L);
+    const Q fm = face_minus<false>(q, s, L);
+    const Q fp = face_plus<false>(q, s, L);
+    const Q fpL = shr(fp), sL = shr(s);
+    IfState W, Wt;
+    grp<false>(fpL, fm, sL, s, G, W, Wt);
+    const Q F = flux<false>(W, Wt, dt);
+    const Q FR = shl(F);
+    const Q qo = update<false>(q, F, FR, lam, G);
+
+    // ---- reverse
+    double dtb = 0.0;
+    Q Ub{0.0, 0.0, 0.0, 0.0};
+    if (upd) {
+        Ub = prim_adj(qo, qbo, G);
+        // U = cons(q) - lam (FR - F), lam = dt / h
+        dtb -= ((FR.r - F.r) * Ub.r + (FR.p - F.p) * Ub.p + (FR.un - F.un) * Ub.un +
+                (FR.ut - F.ut) * Ub.ut) * L.ih;
+    }
+    const Q UbL = shr(Ub);
+    Q fpLb{0.0, 0.0, 0.0, 0.0}, fmb{0.0, 0.0, 0.0, 0.0}, sLb{0.0, 0.0, 0.0, 0.0},
+        sb{0.0, 0.0, 0.0, 0.0};
+    if (ifc) {
+        const Q Fb{lam * (Ub.r - UbL.r), lam * (Ub.p - UbL.p), lam * (Ub.un - UbL.un),
+                   lam * (Ub.ut - UbL.ut)};
+        IfState Wb, Wtb;
+        dtb += flux_adj(W, Wt, dt, Fb, Wb, Wtb);
+        grp_adj(fpL, fm, sL, s, G, Wb, Wtb, fpLb, fmb, sLb, sb);
+    }
+    const Q fpb = shl(fpLb), sbR = shl(sLb);
+    Q qb{fmb.r + fpb.r, fmb.p + fpb.p, fmb.un + fpb.un, fmb.ut + fpb.ut};
+    Q st{0.0, 0.0, 0.0, 0.0};
+    if (sig) {
+        st = Q{sb.r + sbR.r + L.hh * (fpb.r - fmb.r), sb.p + sbR.p + L.hh * (fpb.p - fmb.p),
+               sb.un + sbR.un + L.hh * (fpb.un - fmb.un), sb.ut + sbR.ut + L.hh * (fpb.ut - fmb.ut)};
+    }
+    Q pL, pC, pR;
+    slope_adj(qL, q, qR, L, st, pL, pC, pR);
+    const Q fromR = shl(pL), fromL = shr(pR);
+    const Q cb = upd ? cons_adj(q, Ub, G) : qbo;
+    qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r, qb.p + pC.p + fromR.p + fromL.p + cb.p,
+           qb.un + pC.un + fromR.un + fromL.un + cb.un, qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut};
+
+    double *out = mode == 0 ? (m.ydir ? d.lamA1 : d.lamB1) : d.lam_out;
+    if (own) store_line(out, P, m, qb, mode == 2);
+    const double part = wave_sum(own ? dtb : 0.0);
+    if (lane == 0) d.dtpart[part_base + tk] = part;
+}
+
+// ---------------------------------------------------------------- R4
+// lam[argmax] += (sum of dt sensitivities) * d dt / d U[argmax], where
+// dt = CFL h / s_max and s_max = speed(U[argmax]).  One block per instance;
+// the sum runs in a fixed order (strided partial sums, then a tree), so the
+// result is reproducible.
+__global__ void __launch_bounds__(256) adj_dt_term_kernel(AdjDesc *__restrict__ descs) {
+    __shared__ double red[256];
+    AdjDesc &d = descs[blockIdx.x];
+    double acc = 0.0;
+    for (int r = 0; r < 3; r++)
+        for (int p = d.part_lo[r] + (int)threadIdx.x; p < d.part_hi[r]; p += blockDim.x) acc += d.dtpart[p];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dtb = red[0];
+        const long long k = d.ctl->argmax;
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        const long long P = d.plane;
+        const Q g = speed_grad(d.U[o], d.U[P + o], d.U[2 * P + o], d.U[3 * P + o], d.gas);
+        const double f = dtb * (-d.ctl->dt / d.ctl->smax);
+        d.lam_out[o] += f * g.r;
+        d.lam_out[P + o] += f * g.p;
+        d.lam_out[2 * P + o] += f * g.un;
+        d.lam_out[3 * P + o] += f * g.ut;
+        d.ctl->dtbar = dtb;
+    }
+}
+
+// ---------------------------------------------------------------- boundary conversions
+// API planes are contiguous [4][ny][nx] in the order (rho, rho u, rho v, E);
+// internal planes are pitched (rho, p, u, v).
+__global__ void cons_to_prim_kernel(const double *__restrict__ cons, double *__restrict__ prim, int nx,
+                                    int ny, int pitch, long long plane, double g1) {
+    const long long n = (long long)nx * ny;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = (int)(k / nx), i = (int)(k % nx);
+    const long long o = (long long)j * pitch + i;
+    const double r = cons[k], mu = cons[n + k], mv = cons[2 * n + k], E = cons[3 * n + k];
+    const double u = mu / r, v = mv / r;
+    prim[o] = r;
+    prim[plane + o] = g1 * (E - 0.5 * (mu * u + mv * v));
+    prim[2 * plane + o] = u;
+    prim[3 * plane + o] = v;
+}
+
+__global__ void prim_to_cons_kernel(const double *__restrict__ prim, double *__restrict__ cons, int nx,
+                                    int ny, int pitch, long long plane, double g1i) {
+    const long long n = (long long)nx * ny;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = (int)(k / nx), i = (int)(k % nx);
+    const long long o = (long long)j * pitch + i;
+    const double r = prim[o], p = prim[plane + o], u = prim[2 * plane + o], v = prim[3 * plane + o];
+    cons[k] = r;
+    cons[n + k] = r * u;
+    cons[2 * n + k] = r * v;
+    cons[3 * n + k] = p * g1i + 0.5 * r * (u * u + v * v);
+}
+
+// lam_prim = (d cons / d prim)^T g_cons at the primitive state prim.
+__global__ void cons_grad_to_prim_kernel(const double *__restrict__ prim, const double *__restrict__ g,
+                                         double *__restrict__ lam, int nx, int ny, int pitch,
+                                         long long plane, double g1i) {
+    const long long n = (long long)nx * ny;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = (int)(k / nx), i = (int)(k % nx);
+    const long long o = (long long)j * pitch + i;
+    const double r = prim[o], u = prim[2 * plane + o], v = prim[3 * plane + o];
+    const double gr = g[k], gmu = g[n + k], gmv = g[2 * n + k], gE = g[3 * n + k];
+    lam[o] = gr + u * gmu + v * gmv + 0.5 * (u * u + v * v) * gE;
+    lam[plane + o] = g1i * gE;
+    lam[2 * plane + o] = r * gmu + r * u * gE;
+    lam[3 * plane + o] = r * gmv + r * v * gE;
+}
+
+// g_cons = (d prim / d cons)^T lam_prim at the conserved input state.
+__global__ void prim_grad_to_cons_kernel(const double *__restrict__ cons, const double *__restrict__ lam,
+                                         double *__restrict__ g, int nx, int ny, int pitch,
+                                         long long plane, double g1) {
+    const long long n = (long long)nx * ny;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = (int)(k / nx), i = (int)(k % nx);
+    const long long o = (long long)j * pitch + i;
+    const double r = cons[k], u = cons[n + k] / r, v = cons[2 * n + k] / r;
+    const double lr = lam[o], lp = lam[plane + o], lu = lam[2 * plane + o], lv = lam[3 * plane + o];
+    g[k] = lr - (lu * u + lv * v) / r + lp * g1 * 0.5 * (u * u + v * v);
+    g[n + k] = lu / r - lp * g1 * u;
+    g[2 * n + k] = lv / r - lp * g1 * v;
+    g[3 * n + k] = lp * g1;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s) {
+    hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks, ntasks);
+}
+void adj_launch_smax(AdjDesc *descs, int inst, long long ncells, hipStream_t s) {
+    hipLaunchKernelGGL(adj_smax_kernel, dim3(blocks_for(ncells, 256)), dim3(256), 0, s, descs, inst);
+}
+void adj_launch_dt(AdjDesc *descs, int ninst, hipStream_t s) {
+    hipLaunchKernelGGL(adj_dt_kernel, dim3(1), dim3(256), 0, s, descs, ninst);
+}
+void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(adj_sweep_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks,
+                       ntasks, mode, part_base);
+}
+void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s) {
+    (void)nparts;
+    hipLaunchKernelGGL(adj_dt_term_kernel, dim3(ninst), dim3(256), 0, s, descs);
+}
+void adj_launch_cons_to_prim(const double *cons, double *prim, int nx, int ny, int pitch, long long plane,
+                             double g1, hipStream_t s) {
+    hipLaunchKernelGGL(cons_to_prim_kernel, dim3(blocks_for((long long)nx * ny, 256)), dim3(256), 0, s,
+                       cons, prim, nx, ny, pitch, plane, g1);
+}
+void adj_launch_prim_to_cons(const double *prim, double *cons, int nx, int ny, int pitch, long long plane,
+                             double g1i, hipStream_t s) {
+    hipLaunchKernelGGL(prim_to_cons_kernel, dim3(blocks_for((long long)nx * ny, 256)), dim3(256), 0, s,
+                       prim, cons, nx, ny, pitch, plane, g1i);
+}
+void adj_launch_cons_grad_to_prim(const double *prim, const double *g, double *lam, int nx, int ny,
+                                  int pitch, long long plane, double g1i, hipStream_t s) {
+    hipLaunchKernelGGL(cons_grad_to_prim_kernel, dim3(blocks_for((long long)nx * ny, 256)), dim3(256), 0,
+                       s, prim, g, lam, nx, ny, pitch, plane, g1i);
+}
+void adj_launch_prim_grad_to_cons(const double *cons, const double *lam, double *g, int nx, int ny,
+                                  int pitch, long long plane, double g1, hipStream_t s) {
+    hipLaunchKernelGGL(prim_grad_to_cons_kernel, dim3(blocks_for((long long)nx * ny, 256)), dim3(256), 0,
+                       s, cons, lam, g, nx, ny, pitch, plane, g1);
+}
+
+}  // namespace fvm2d

@@ -1,0 +1,498 @@
+// Host-side engine behind the C ABI: device buffers of n instances, the
+// device-resident time loop, launch geometry (with an autotuner) and hipGraph
+// capture of step chunks.  Shared by the forward and adjoint front ends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm2d.h"
+#include "fvm2d_device.h"
+#include "fvm2d_kernels.h"
+
+namespace fvm2d {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(FVM2D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+inline int check_params(int nx, int ny, const fvm2d_params *p) {
+    if (!p) return fail(FVM2D_ERR_ARG, "params is null");
+    if (nx < 6 || ny < 6)
+        return fail(FVM2D_ERR_ARG, "mesh must be at least 6x6 cells (two ghost layers per side)");
+    if ((long long)nx * ny > (1ll << 31) - 1)
+        return fail(FVM2D_ERR_ARG, "mesh has more than 2^31-1 cells");
+    if (!(p->width > 0.0) || !(p->height > 0.0)) return fail(FVM2D_ERR_ARG, "width/height must be > 0");
+    if (!(p->cfl > 0.0)) return fail(FVM2D_ERR_ARG, "CFL must be > 0");
+    if (!(p->gamma > 1.0)) return fail(FVM2D_ERR_ARG, "gamma must be > 1");
+    if (std::isnan(p->t_end) || std::isnan(p->alpha)) return fail(FVM2D_ERR_ARG, "NaN parameter");
+    if (p->strategy < 0 || p->strategy > 4) return fail(FVM2D_ERR_ARG, "unknown ghost-cell strategy");
+    if (p->mode != FVM2D_MODE_FAST && p->mode != FVM2D_MODE_EXACT)
+        return fail(FVM2D_ERR_ARG, "unknown arithmetic mode");
+    return FVM2D_OK;
+}
+
+inline int device_ok() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return fail(FVM2D_ERR_NODEVICE, "no HIP device available (libfvm2d has no CPU fallback)");
+    return FVM2D_OK;
+}
+
+// --------------------------------------------------------------------------
+// Engine: n instances advanced by one step kernel per time step.
+struct Engine {
+    int n = 0;
+    bool exact = false;
+    std::vector<fvm2d_params> prm;
+    std::vector<InstDesc> hd;        // host copies of the descriptors
+    std::vector<long long> buf_off;  // offsets (doubles) of each instance's buffers in the pool
+    double *pool = nullptr;
+    long long pool_doubles = 0;
+    InstDesc *d_desc = nullptr;
+    Ctl *d_ctl = nullptr;
+    Ctl *h_ctl = nullptr;            // pinned read-back
+    Sched *d_sched = nullptr;
+    double *d_t0 = nullptr, *h_t0 = nullptr;  // per-instance start times
+    int2 *d_tasks = nullptr;
+    int ntasks = 0;
+    int R = 16, wpb = 4, persistent = 0, grid = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;                  // steps captured in gexec
+    double *stage = nullptr;         // pinned staging for one instance
+    long long stage_doubles = 0;
+    std::vector<int> steps_total;
+    bool begun = false;
+
+    ~Engine() { release(); }
+
+    void release() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        if (pool) hipFree(pool);
+        if (d_desc) hipFree(d_desc);
+        if (d_ctl) hipFree(d_ctl);
+        if (d_sched) hipFree(d_sched);
+        if (d_tasks) hipFree(d_tasks);
+        if (h_ctl) hipHostFree(h_ctl);
+        if (d_t0) hipFree(d_t0);
+        if (h_t0) hipHostFree(h_t0);
+        d_t0 = nullptr;
+        h_t0 = nullptr;
+        if (stage) hipHostFree(stage);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (stream) hipStreamDestroy(stream);
+        pool = nullptr;
+        d_desc = nullptr;
+        d_ctl = nullptr;
+        d_sched = nullptr;
+        d_tasks = nullptr;
+        h_ctl = nullptr;
+        stage = nullptr;
+        e0 = e1 = nullptr;
+        stream = nullptr;
+    }
+
+    static void fill_desc(InstDesc &d, int nx, int ny, const fvm2d_params &p) {
+        d.nx = nx;
+        d.ny = ny;
+        d.pitch = ((nx + 15) / 16) * 16;
+        d.plane = (long long)d.pitch * ny;
+        d.nw = (nx - 4 + kOwn - 1) / kOwn;
+        d.strategy = p.strategy;
+        d.max_steps = p.max_steps;
+        d.task_base = 0;
+        d.pad0 = 0;
+        d.t_end = p.t_end;
+        d.cfl = p.cfl;
+        // h = std::min(width / Nx, height / Ny), src/wrappers.cuh:202
+        d.hmin = std::min(p.width / double(nx), p.height / double(ny));
+        // h = width / double(Nx) in MUSCL_rel_RP_GRP and forward_x_dir; the
+        // Y-sweep runs them with (Ny, Nx, height), src/wrappers.cuh:219,223
+        d.lx = make_line(p.width / double(nx), p.alpha);
+        d.ly = make_line(p.height / double(ny), p.alpha);
+        d.gas = make_gas(p.gamma);
+    }
+
+    int init(int count, const int *nx, const int *ny, const fvm2d_params *p) {
+        int rc = device_ok();
+        if (rc) return rc;
+        if (count < 1) return fail(FVM2D_ERR_ARG, "instance count must be >= 1");
+        n = count;
+        prm.assign(p, p + count);
+        exact = p[0].mode == FVM2D_MODE_EXACT;
+        for (int i = 0; i < n; i++) {
+            rc = check_params(nx[i], ny[i], &p[i]);
+            if (rc) return rc;
+            if ((p[i].mode == FVM2D_MODE_EXACT) != exact)
+                return fail(FVM2D_ERR_ARG, "all instances of a batch must use the same mode");
+        }
+        hd.resize(n);
+        buf_off.resize(n);
+        steps_total.assign(n, 0);
+        long long off = 0, maxplane = 0;
+        for (int i = 0; i < n; i++) {
+            fill_desc(hd[i], nx[i], ny[i], p[i]);
+            buf_off[i] = off;
+            off += 8 * hd[i].plane + 64;  // two buffers of four planes, 512-B separation
+            off = (off + 63) / 64 * 64;
+            maxplane = std::max(maxplane, hd[i].plane);
+        }
+        pool_doubles = off;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipMalloc(&pool, pool_doubles * sizeof(double)));
+        HIPCHK(hipMemsetAsync(pool, 0, pool_doubles * sizeof(double), stream));
+        for (int i = 0; i < n; i++) {
+            hd[i].buf0 = pool + buf_off[i];
+            hd[i].buf1 = hd[i].buf0 + 4 * hd[i].plane;
+        }
+        HIPCHK(hipMalloc(&d_desc, n * sizeof(InstDesc)));
+        HIPCHK(hipMalloc(&d_ctl, n * sizeof(Ctl)));
+        HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
+        HIPCHK(hipMalloc(&d_sched, sizeof(Sched)));
+        HIPCHK(hipMemsetAsync(d_sched, 0, sizeof(Sched), stream));
+        HIPCHK(hipHostMalloc(&h_ctl, n * sizeof(Ctl), hipHostMallocDefault));
+        HIPCHK(hipMalloc(&d_t0, n * sizeof(double)));
+        HIPCHK(hipHostMalloc(&h_t0, n * sizeof(double), hipHostMallocDefault));
+        stage_doubles = 4 * maxplane;
+        HIPCHK(hipHostMalloc(&stage, stage_doubles * sizeof(double), hipHostMallocDefault));
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        cu_count = prop.multiProcessorCount;
+        R = default_strip_rows();
+        return set_launch(R, 4, n > 1 ? 1 : 0);
+    }
+
+    int default_strip_rows() const {
+        // aim at roughly two resident waves per SIMD over the whole batch
+        long long cells = 0;
+        for (const auto &d : hd) cells += (long long)(d.nx - 4) * (d.ny - 4);
+        const double target_waves = 2.0 * 4 * cu_count;
+        const double rows = cells / (double)kOwn / target_waves;
+        int r = (int)std::lround(rows);
+        return std::max(8, std::min(64, r));
+    }
+
+    int set_launch(int strip_rows, int waves_per_block, int persist) {
+        if (strip_rows > 0) R = strip_rows;
+        if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
+        if (persist >= 0) persistent = persist ? 1 : 0;
+        if (R < 1) return fail(FVM2D_ERR_ARG, "strip rows must be >= 1");
+        std::vector<int2> tasks;
+        for (int i = 0; i < n; i++) {
+            InstDesc &d = hd[i];
+            d.R = R;
+            d.ns = (d.ny - 4 + R - 1) / R;
+            d.task_base = (int)tasks.size();
+            const int cnt = d.nw * d.ns;
+            for (int k = 0; k < cnt; k++) tasks.push_back(make_int2(i, k));
+        }
+        ntasks = (int)tasks.size();
+        if (d_tasks) HIPCHK(hipFree(d_tasks));
+        d_tasks = nullptr;
+        HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int2)));
+        HIPCHK(hipMemcpyAsync(d_tasks, tasks.data(), ntasks * sizeof(int2), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, stream));
+        const int blocks_needed = (ntasks + wpb - 1) / wpb;
+        if (persistent) {
+            int occ = step_kernel_occupancy(exact, wpb);
+            if (occ < 1) occ = 1;
+            grid = std::min(blocks_needed, occ * cu_count);
+        } else {
+            grid = blocks_needed;
+        }
+        if (grid < 1) grid = 1;
+        drop_graph();
+        HIPCHK(hipStreamSynchronize(stream));
+        return FVM2D_OK;
+    }
+
+    void drop_graph() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gsteps = 0;
+    }
+
+    int upload_desc() {
+        HIPCHK(hipMemcpyAsync(d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, stream));
+        return FVM2D_OK;
+    }
+
+    // Host vec5d [ny][nx][5] -> current-state buffer (buf0) of instance i.
+    int upload(int i, const double *U5) {
+        if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
+        const InstDesc &d = hd[i];
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int k = 0; k < 4; k++)
+            for (int j = 0; j < d.ny; j++) {
+                double *dst = stage + k * d.plane + (long long)j * d.pitch;
+                const double *s = U5 + ((long long)j * d.nx) * 5 + k;
+                for (int c = 0; c < d.nx; c++) dst[c] = s[5 * c];
+                for (int c = d.nx; c < d.pitch; c++) dst[c] = 0.0;
+            }
+        HIPCHK(hipMemcpyAsync(d.buf0, stage, 4 * d.plane * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        begun = false;
+        return FVM2D_OK;
+    }
+
+    int download(int i, double *U5) {
+        if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
+        const InstDesc &d = hd[i];
+        HIPCHK(hipMemcpyAsync(stage, d.buf0, 4 * d.plane * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int j = 0; j < d.ny; j++) {
+            double *o = U5 + (long long)j * d.nx * 5;
+            for (int c = 0; c < d.nx; c++) {
+                for (int k = 0; k < 4; k++) o[5 * c + k] = stage[k * d.plane + (long long)j * d.pitch + c];
+                o[5 * c + 4] = 0.0;
+            }
+        }
+        return FVM2D_OK;
+    }
+
+    int planes_device(int i, double *d_planes, bool in) {
+        if (i < 0 || i >= n || !d_planes) return fail(FVM2D_ERR_ARG, "bad instance or null pointer");
+        const InstDesc &d = hd[i];
+        for (int k = 0; k < 4; k++) {
+            double *a = d.buf0 + k * d.plane;
+            double *b = d_planes + (long long)k * d.nx * d.ny;
+            if (in)
+                HIPCHK(hipMemcpy2DAsync(a, d.pitch * sizeof(double), b, d.nx * sizeof(double),
+                                        d.nx * sizeof(double), d.ny, hipMemcpyDeviceToDevice, stream));
+            else
+                HIPCHK(hipMemcpy2DAsync(b, d.nx * sizeof(double), a, d.pitch * sizeof(double),
+                                        d.nx * sizeof(double), d.ny, hipMemcpyDeviceToDevice, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        if (in) begun = false;
+        return FVM2D_OK;
+    }
+
+    // The state of every instance is in buf0: reset its time loop to t = 0
+    // and compute s_max of the initial state.
+    // t0: per-instance start times (host array of n values), or null for 0.
+    int begin(const double *t0 = nullptr) {
+        HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
+        for (int i = 0; i < n; i++)
+            launch_speed(exact, d_desc, d_ctl, i, 0, (long long)hd[i].nx * hd[i].ny, stream);
+        if (t0) {
+            for (int i = 0; i < n; i++) h_t0[i] = t0[i];
+            HIPCHK(hipMemcpyAsync(d_t0, h_t0, n * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        launch_ctl_start(d_desc, d_ctl, n, t0 ? d_t0 : nullptr, stream);
+        HIPCHK(hipGetLastError());
+        begun = true;
+        return FVM2D_OK;
+    }
+
+    void launch_one() {
+        launch_step(exact, d_desc, d_ctl, d_tasks, ntasks, d_sched, persistent, grid, wpb, stream);
+    }
+
+    int ensure_graph(int steps) {
+        if (gexec && gsteps == steps) return FVM2D_OK;
+        drop_graph();
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < steps; k++) launch_one();
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            gexec = nullptr;
+            return fail(FVM2D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        gsteps = steps;
+        return FVM2D_OK;
+    }
+
+    static constexpr int kGraphSteps = 16;
+
+    // Enqueue nsteps step launches (graph replays + a remainder of plain launches).
+    int enqueue(int nsteps) {
+        int rc = ensure_graph(kGraphSteps);
+        if (rc) return rc;
+        int k = 0;
+        for (; k + kGraphSteps <= nsteps; k += kGraphSteps) HIPCHK(hipGraphLaunch(gexec, stream));
+        for (; k < nsteps; k++) launch_one();
+        HIPCHK(hipGetLastError());
+        return FVM2D_OK;
+    }
+
+    int read_ctl() {
+        HIPCHK(hipMemcpyAsync(h_ctl, d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return FVM2D_OK;
+    }
+
+    bool all_done() const {
+        for (int i = 0; i < n; i++)
+            if (!h_ctl[i].done) return false;
+        return true;
+    }
+
+    // Steps still needed by the slowest instance, estimated from its dt.
+    int remaining_estimate() const {
+        long long best = 0;
+        for (int i = 0; i < n; i++) {
+            const Ctl &c = h_ctl[i];
+            if (c.done) continue;
+            long long rem = 1ll << 30;
+            if (prm[i].max_steps >= 0) rem = std::max(0, prm[i].max_steps - c.steps);
+            if (std::isfinite(prm[i].t_end) && c.dt > 0.0) {
+                const double r = (prm[i].t_end - c.t) / c.dt;
+                rem = std::min(rem, (long long)std::ceil(r * 1.02) + 2);
+            }
+            best = std::max(best, rem);
+        }
+        return (int)std::min<long long>(best, 1 << 20);
+    }
+
+    // Device-resident time loop until every instance is done.
+    int run_to_completion(double *ms_out) {
+        if (!begun) {
+            int rc = begin();
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(e0, stream));
+        int rc = read_ctl();
+        if (rc) return rc;
+        int chunk = 8;
+        while (!all_done()) {
+            rc = enqueue(chunk);
+            if (rc) return rc;
+            rc = read_ctl();
+            if (rc) return rc;
+            chunk = std::max(1, remaining_estimate());
+        }
+        HIPCHK(hipEventRecord(e1, stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_out) *ms_out = ms;
+        return finish_run();
+    }
+
+    // Time every candidate launch geometry on a scratch copy of the states
+    // (t_end lifted so no instance stops) and keep the fastest.
+    int autotune(int steps_per_trial, double *best_ms_out) {
+        if (steps_per_trial < 1) steps_per_trial = 4;
+        long long save_doubles = 0;
+        for (int i = 0; i < n; i++) save_doubles += 4 * hd[i].plane;
+        double *save = nullptr;
+        HIPCHK(hipMalloc(&save, save_doubles * sizeof(double)));
+        auto copy_states = [&](bool to_save) -> int {
+            long long o = 0;
+            for (int i = 0; i < n; i++) {
+                const long long bytes = 4 * hd[i].plane * sizeof(double);
+                if (to_save) HIPCHK(hipMemcpyAsync(save + o, hd[i].buf0, bytes, hipMemcpyDeviceToDevice, stream));
+                else HIPCHK(hipMemcpyAsync(hd[i].buf0, save + o, bytes, hipMemcpyDeviceToDevice, stream));
+                o += 4 * hd[i].plane;
+            }
+            return FVM2D_OK;
+        };
+        int rc = copy_states(true);
+        std::vector<double> tend(n);
+        std::vector<int> msteps(n);
+        for (int i = 0; i < n; i++) {
+            tend[i] = hd[i].t_end;
+            msteps[i] = hd[i].max_steps;
+            hd[i].t_end = std::numeric_limits<double>::infinity();
+            hd[i].max_steps = -1;
+        }
+        int maxny = 0;
+        for (int i = 0; i < n; i++) maxny = std::max(maxny, hd[i].ny - 4);
+        const int Rs[] = {4, 8, 12, 16, 24, 32, 48, 64};
+        double best = std::numeric_limits<double>::infinity();
+        int bR = R, bW = wpb, bP = persistent;
+        for (int r : Rs) {
+            if (rc) break;
+            if (r > std::max(4, maxny)) continue;
+            for (int w : {1, 2, 4}) {
+                for (int pers : {0, 1}) {
+                    rc = set_launch(r, w, pers);
+                    if (!rc) rc = copy_states(false);
+                    if (!rc) rc = begin();
+                    if (!rc) rc = enqueue(2);
+                    if (rc) break;
+                    hipEventRecord(e0, stream);
+                    rc = enqueue(steps_per_trial);
+                    if (rc) break;
+                    hipEventRecord(e1, stream);
+                    if (hipEventSynchronize(e1) != hipSuccess) {
+                        rc = fail(FVM2D_ERR_HIP, "autotune trial failed");
+                        break;
+                    }
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    const double per = ms / steps_per_trial;
+                    if (per < best) {
+                        best = per;
+                        bR = r;
+                        bW = w;
+                        bP = pers;
+                    }
+                }
+                if (rc) break;
+            }
+        }
+        for (int i = 0; i < n; i++) {
+            hd[i].t_end = tend[i];
+            hd[i].max_steps = msteps[i];
+        }
+        int rc2 = set_launch(bR, bW, bP);
+        if (!rc2) rc2 = copy_states(false);
+        hipStreamSynchronize(stream);
+        hipFree(save);
+        begun = false;
+        if (best_ms_out) *best_ms_out = best;
+        return rc ? rc : rc2;
+    }
+
+    // After a run: move each final state to buf0 (swap buffers when the
+    // step count is odd) so that the next run continues from it.
+    int finish_run() {
+        int rc = read_ctl();
+        if (rc) return rc;
+        int status = 0;
+        for (int i = 0; i < n; i++) {
+            if (h_ctl[i].steps & 1) std::swap(hd[i].buf0, hd[i].buf1);
+            steps_total[i] += h_ctl[i].steps;
+            status |= h_ctl[i].status;
+        }
+        rc = upload_desc();
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(stream));
+        begun = false;
+        if (status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite wave speed (NaN state) reached");
+        return FVM2D_OK;
+    }
+};
+
+}  // namespace fvm2d

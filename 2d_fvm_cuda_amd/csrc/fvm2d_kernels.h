@@ -11,7 +11,8 @@ void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream);
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
                   long long ncells, hipStream_t stream);
-void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, hipStream_t stream);
+void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double *t0s,
+                      hipStream_t stream);
 int step_kernel_occupancy(bool exact, int wpb);
 
 }  // namespace fvm2d

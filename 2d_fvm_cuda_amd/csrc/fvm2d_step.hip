@@ -24,49 +24,10 @@
 
 #include "fvm2d_device.h"
 #include "fvm2d_kernels.h"
+#include "fvm2d_ghost.h"
+#include "fvm2d_lanes.h"
 
 namespace fvm2d {
-
-// ---------------------------------------------------------------- lanes
-__device__ __forceinline__ double dpp_shr1(double x) {  // lane l <- lane l-1
-    int lo = __double2loint(x), hi = __double2hiint(x);
-    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double dpp_shl1(double x) {  // lane l <- lane l+1
-    int lo = __double2loint(x), hi = __double2hiint(x);
-    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ Q shr(const Q &q) {
-    return Q{dpp_shr1(q.r), dpp_shr1(q.p), dpp_shr1(q.un), dpp_shr1(q.ut)};
-}
-__device__ __forceinline__ Q shl(const Q &q) {
-    return Q{dpp_shl1(q.r), dpp_shl1(q.p), dpp_shl1(q.un), dpp_shl1(q.ut)};
-}
-
-// X-order (rho, p, u, v) <-> Y-line order (rho, p, v, u)
-__device__ __forceinline__ Q swap_uv(const Q &q) { return Q{q.r, q.p, q.ut, q.un}; }
-
-// ---------------------------------------------------------------- sweeps
-// One X-sweep of a row held across the wave, lanes = consecutive columns.
-// Output is valid on lanes 2..61 (a two-lane stencil halo each side).
-template <bool Exact>
-__device__ __forceinline__ Q xsweep(const Q &q, const Line &L, const Gas &G, double dt,
-                                    double lam) {
-    const Q qL = shr(q), qR = shl(q);
-    const Q s = slope<Exact>(qL, q, qR, L);
-    const Q fm = face_minus<Exact>(q, s, L);
-    const Q fp = face_plus<Exact>(q, s, L);
-    const Q fpL = shr(fp), sL = shr(s);
-    IfState W, Wt;
-    grp<Exact>(fpL, fm, sL, s, G, W, Wt);  // interface on the west face
-    const Q F = flux<Exact>(W, Wt, dt);
-    const Q FR = shl(F);                   // interface on the east face
-    return update<Exact>(q, F, FR, lam, G);
-}
 
 // Register pipeline of a Y-sweep marching south (increasing row j).  In the
 // sweep's line order the "left" neighbour of row j is row j+1, which is the
@@ -111,69 +72,6 @@ __device__ __forceinline__ Q ymarch_step(YMarch &m, const Q &qn, const Line &L, 
     m.fmj = fm1;
     m.Fprev = F;
     return out;
-}
-
-// ---------------------------------------------------------------- ghosts
-struct GhostSrc {
-    int is_const;
-    int sj, si;
-    int negv;
-    double r, p, u, v;
-};
-
-__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
-
-// Value of ghost cell (gj, gi) after the preset's two-phase fill, as a copy
-// of an interior cell (optionally with v negated) or a constant state.
-__device__ GhostSrc ghost_source(int strategy, int gj, int gi, int nx, int ny, double t) {
-    GhostSrc g{0, clampi(gj, 2, ny - 3), clampi(gi, 2, nx - 3), 0, 0.0, 0.0, 0.0, 0.0};
-    auto set_const = [&](double r, double p, double u, double v) {
-        g.is_const = 1;
-        g.r = r;
-        g.p = p;
-        g.u = u;
-        g.v = v;
-    };
-    if (strategy == kOutflow) return g;  // ghostCellPresets.cuh:106-145
-    if (strategy == kJet) {              // ghostCellPresets.cuh:148-193
-        if (gi <= 1) {
-            if (gj >= 2 * ny / 5 + 1 && gj <= 3 * ny / 5) set_const(5.0, 0.4127, 800.0, 0.0);
-            else set_const(0.5, 0.4127, 0.0, 0.0);
-        }
-        return g;
-    }
-    if (strategy == kJet800) {  // ghostCellPresets.cuh:196-241
-        if (gi <= 1) {
-            if (gj >= 9 * ny / 10) set_const(1.4, 1.0, 800.0, 0.0);
-            else set_const(0.14, 1.0, 0.0, 0.0);
-        } else if (gj == ny - 1) {
-            g.sj = ny - 4;
-        }
-        return g;
-    }
-    // double Mach (also selected by "reflective"), ghostCellPresets.cuh:48-103
-    const double post_u = 4.125 * sqrt(3.0);
-    if (gi <= 1) {
-        set_const(8.0, 116.5, post_u, -4.125);
-    } else if (gi >= nx - 2) {
-        set_const(1.4, 1.0, 0.0, 0.0);
-    } else if (gj <= 1) {
-        double temp = double(ny) / sqrt(3.0);
-        double dist = double(ny) * 20 * t / sqrt(3.0);
-        if (gi <= nx / 24 + int(temp) + int(dist)) set_const(8.0, 116.5, post_u, -4.125);
-        else set_const(1.4, 1.0, 0.0, 0.0);
-    } else if (gj == ny - 2) {
-        g.sj = ny - 3;
-        g.negv = gi > nx / 24;
-    } else {  // gj == ny - 1
-        if (gi <= nx / 24) {
-            g.sj = ny - 3;
-        } else {
-            g.sj = ny - 4;
-            g.negv = 1;
-        }
-    }
-    return g;
 }
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
@@ -391,15 +289,15 @@ __global__ void __launch_bounds__(256) fvm_speed_kernel(const InstDesc *__restri
 }
 
 __global__ void fvm_ctl_start_kernel(const InstDesc *__restrict__ descs, Ctl *__restrict__ ctls,
-                                     int ninst) {
+                                     int ninst, const double *__restrict__ t0s) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     Ctl *c = ctls + i;
     const unsigned long long sb = c->smax_next;
-    // the reference's loop test happens before the first step: t = 0
+    // the reference's loop test happens before the first step (t = t0 = 0)
     const InstDesc d = descs[i];
     c->status = 0;
-    advance_ctl(d, c, 0.0, 0, sb);
+    advance_ctl(d, c, t0s ? t0s[i] : 0.0, 0, sb);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -427,9 +325,10 @@ void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int us
                            descs, ctls, inst, use_buf1);
 }
 
-void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, hipStream_t stream) {
+void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double *t0s,
+                      hipStream_t stream) {
     hipLaunchKernelGGL(fvm_ctl_start_kernel, dim3((ninst + 63) / 64), dim3(64), 0, stream, descs,
-                       ctls, ninst);
+                       ctls, ninst, t0s);
 }
 
 int step_kernel_occupancy(bool exact, int wpb) {

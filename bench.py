@@ -1,0 +1,152 @@
+"""Benchmark: FP64 cell-updates/s of the 2000x2000 four-quadrant solve on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extras]
+
+Workload: the four-quadrant Riemann problem (test2D.cu:46-80) on a
+2000 x 2000 FP64 mesh (two ghost layers included), [0,1]^2, gamma 1.4,
+CFL 0.5, alpha 1.9, outflow ghosts, to t_end = 0.3 (about 3,000 time steps).
+A "step" is one full time step of the split scheme over the whole mesh.
+The state is uploaded before the timed region; the timed region is the
+device-resident time loop bracketed by HIP events on the solver's stream
+(the region of src/wrappers.cuh:193-288).  --steps K > 0 caps the timed loop
+at K steps; the default (0) runs the whole solve to t_end.
+
+Printed: one JSON line with the headline metric, the roofline record of the
+step kernel and the CPU baseline (the FP64 host oracle on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+BYTES_PER_CELL_UPDATE = 288  # 4 sweeps x (32 B read + 32 B write) + 32 B partner read
+
+
+def cpu_baseline(U0, steps: int, threads: int):
+    """The oracle (FP64 OpenMP host restatement) on the first ``steps`` steps
+    of the same workload."""
+    import oracle
+    _, n, _, ms = oracle.solve(U0, nthreads=threads, t_end=0.3, max_steps=steps)
+    cells = U0.shape[0] * U0.shape[1]
+    return cells * n / (ms / 1000.0), n, ms
+
+
+def traffic_record():
+    path = os.path.join(ROOT, "profiles", "step_kernel_2000_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = full solve to t_end)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=2000)
+    ap.add_argument("--mode", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--extras", action="store_true", help="also time the batched and adjoint paths")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the y-strip multi-GPU decomposition is a design, not code", file=sys.stderr)
+        sys.exit(2)
+
+    fvm = importlib.import_module("2d_fvm_cuda_amd")
+    problems = fvm.problems
+    if not fvm.device_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    N = args.n
+    mode = fvm.MODE_EXACT if args.mode == "exact" else fvm.MODE_FAST
+    U0 = problems.initialize_4shocks(N)
+    prm = dict(t_end=0.3, cfl=0.5, alpha=1.9, gamma=1.4, strategy=0, mode=mode,
+               max_steps=(args.steps if args.steps > 0 else -1))
+    s = fvm.Solver(N, N, **prm)
+    s.upload(U0)
+    tuned_ms = None
+    if not args.no_autotune:
+        tuned_ms = s.autotune(steps_per_trial=4)
+    launch = s.get_launch()
+
+    # warm-up: W steps from the initial data, then restore it
+    if args.warmup > 0:
+        s.begin()
+        s.advance(args.warmup)
+        s.sync()
+        s.upload(U0)
+
+    steps, t_final, ms = s.run()
+    cells = N * N
+    value = cells * steps / (ms / 1000.0)
+
+    # live per-launch duration of the step kernel (HIP events on its stream),
+    # on a fresh copy of the same workload
+    s.upload(U0)
+    s.begin()
+    k_avg_ms, k_min_ms = s.time_kernel(50)
+    s.close()
+
+    achieved = BYTES_PER_CELL_UPDATE * cells / (k_avg_ms / 1000.0) / 1e9
+    out = {
+        "metric": "FP64 cell-updates/s at 2000^2 (% HBM roofline); batched and adjoint vs plain",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / max(steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"four-quadrant Riemann problem {N}x{N} FP64 to t_end 0.3 "
+                               "(CFL 0.5, alpha 1.9, gamma 1.4, outflow)",
+                   "mesh": [N, N], "t_end": 0.3, "t_final": t_final, "mode": args.mode,
+                   "launch": {"strip_rows": launch[0], "waves_per_block": launch[1],
+                              "persistent": launch[2]}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_record(),
+                     "kernel": "fvm_step_kernel", "kernel_avg_ms": k_avg_ms,
+                     "kernel_min_ms": k_min_ms,
+                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
+    }
+    if tuned_ms is not None:
+        out["config"]["autotune_best_ms_per_step"] = tuned_ms
+
+    if args.extras:
+        from importlib import import_module
+        extras = import_module("2d_fvm_cuda_amd.benchmarks")
+        out["batched"] = extras.batched(fvm, plain_rate=None)
+        out["adjoint"] = extras.adjoint(fvm)
+
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        t0 = time.time()
+        v, n, cms = cpu_baseline(U0, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle, first {n} steps of the same {N}x{N} workload "
+                                         f"({cms / 1000.0:.1f} s wall)"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

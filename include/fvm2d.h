@@ -110,6 +110,9 @@ int fvm2d_sync(fvm2d_solver *s, int *steps_out, double *t_out, double *dt_out);
 /* Wall time of nsteps launches bracketed by HIP events on the solver's
  * stream (after fvm2d_begin).  ms_out: total milliseconds. */
 int fvm2d_time_steps(fvm2d_solver *s, int nsteps, double *ms_out);
+/* Duration of single step-kernel launches: nsteps launches, each bracketed
+ * by its own pair of HIP events on the solver's stream.  avg/min in ms. */
+int fvm2d_time_kernel(fvm2d_solver *s, int nsteps, double *avg_ms_out, double *min_ms_out);
 
 /* ---- batched forward (no counterpart in the reference) ----------------- *
  * n independent meshes of their own size and parameters advance together:
@@ -129,6 +132,34 @@ int fvm2d_batch_download(fvm2d_batch *b, int i, double *U5);
 int fvm2d_batch_set_launch(fvm2d_batch *b, int strip_rows, int waves_per_block, int persistent);
 int fvm2d_batch_autotune(fvm2d_batch *b, int steps_per_trial, double *best_ms_out);
 int fvm2d_batch_destroy(fvm2d_batch *b);
+
+/* ---- adjoint (no counterpart in the reference) ------------------------- *
+ * Hand-written reverse mode of the full split step (limiter and GRP
+ * differentiated along the branch taken, dt differentiated through its
+ * arg-max cell) over nsteps fixed steps dt = CFL h / s_max (t_end is not
+ * used), with binomial time-step checkpointing in checkpoint_slots states
+ * per instance (slot 0 holds the initial state; <= 0 stores every step).
+ * States and gradients cross the boundary as device pointers to contiguous
+ * [4][ny][nx] conserved planes in the order (rho, rho u, rho v, E); one
+ * pointer per instance.  Arithmetic is the FAST mode. */
+typedef struct fvm2d_adj fvm2d_adj;
+typedef struct {
+    double forward_ms;       /* the forward call (with its checkpoint copies) */
+    double backward_ms;      /* the backward call (reverse steps + recomputation) */
+    double forward_step_ms;  /* forward_ms / nsteps */
+    double reverse_step_ms;  /* mean duration of one reverse step */
+    int reverse_steps;
+    int recompute_steps;     /* forward steps re-run by the backward call */
+    int checkpoint_slots;
+    int schedule_cost;       /* forward steps of the schedule after the stored state */
+} fvm2d_adj_stats;
+int fvm2d_adj_create(int n, const int *nx, const int *ny, const fvm2d_params *p, int nsteps,
+                     int checkpoint_slots, fvm2d_adj **out);
+int fvm2d_adj_forward(fvm2d_adj *a, const double *const *d_U0_cons, double *const *d_UN_cons);
+int fvm2d_adj_backward(fvm2d_adj *a, const double *const *d_gN_cons, double *const *d_g0_cons);
+int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *out);
+int fvm2d_adj_set_launch(fvm2d_adj *a, int strip_rows, int waves_per_block, int persistent);
+int fvm2d_adj_destroy(fvm2d_adj *a);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,10 @@ def lib():
         L.orc_sweep_x.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_rotate.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_adj_forward.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcParams),
+                                      ctypes.c_int, dp]
+        L.orc_adj_backward.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcParams),
+                                       ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -157,4 +161,23 @@ def rotate(U5: np.ndarray) -> np.ndarray:
     ny, nx = U.shape[0], U.shape[1]
     out = np.zeros((nx, ny, 5))
     lib().orc_rotate(_dp(U), _dp(out), nx, ny)
+    return out
+
+
+def adj_forward(U0cons: np.ndarray, nsteps: int, **kw) -> np.ndarray:
+    """Host forward of the differentiated solve: [4][ny][nx] conserved planes."""
+    U = np.ascontiguousarray(U0cons, dtype=np.float64)
+    out = np.zeros_like(U)
+    p = params(**kw)
+    lib().orc_adj_forward(_dp(U), U.shape[2], U.shape[1], ctypes.byref(p), int(nsteps), _dp(out))
+    return out
+
+
+def adj_backward(U0cons: np.ndarray, nsteps: int, gN: np.ndarray, **kw) -> np.ndarray:
+    """Host reference gradient dL/dU0 given dL/dU_N (conserved planes)."""
+    U = np.ascontiguousarray(U0cons, dtype=np.float64)
+    g = np.ascontiguousarray(gN, dtype=np.float64)
+    out = np.zeros_like(U)
+    p = params(**kw)
+    lib().orc_adj_backward(_dp(U), U.shape[2], U.shape[1], ctypes.byref(p), int(nsteps), _dp(g), _dp(out))
     return out

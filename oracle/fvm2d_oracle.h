@@ -62,6 +62,14 @@ void orc_sweep_x(double *U5, int nx, int ny, double width, double dt,
 /* Anti-transpose with u<->v swap: out is [nx][ny][5]. */
 void orc_rotate(const double *in5, double *out5, int nx, int ny);
 
+/* Host reference of the differentiated solve (fvm2d_oracle_adjoint.cpp):
+ * nsteps steps with dt = CFL h / s_max, FAST per-cell numerics; states and
+ * gradients are conserved planes [4][ny][nx] (rho, rho u, rho v, E). */
+int orc_adj_forward(const double *U0cons, int nx, int ny, const orc_params *p, int nsteps,
+                    double *UNcons);
+int orc_adj_backward(const double *U0cons, int nx, int ny, const orc_params *p, int nsteps,
+                     const double *gNcons, double *g0cons);
+
 #ifdef __cplusplus
 }
 #endif
