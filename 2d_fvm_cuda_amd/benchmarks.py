@@ -1,0 +1,135 @@
+"""Benchmark legs beyond the plain 2000^2 solve (used by bench.py).
+
+batched()          64 seeded four-quadrant meshes, sides ~U{128..1024}, 200 steps
+                   each in one persistent launch per step (hipGraph-captured),
+                   against the same meshes solved one after another.
+adjoint()          512^2 Fourier-perturbed state, 400 steps forward with
+                   binomial checkpointing, then the reverse sweep; reports the
+                   reverse-step / forward-step time ratio and an FD spot check.
+batched_adjoint()  32 meshes with sides ~U{256..768} (the adjoint recipe under
+                   seeds 1..32), 300 steps forward + backward in one batch.
+peak()             8192^2 plain forward (2.1 GB of state), 20 timed steps.
+Cell-update counts use the full meshes (ghost layers included).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+BYTES_PER_CELL_UPDATE = 288
+HBM_PEAK_GBS = 8000.0
+G = 1.4
+
+
+def _cons(U5):
+    r, p, u, v = U5[..., 0], U5[..., 1], U5[..., 2], U5[..., 3]
+    return np.ascontiguousarray(np.stack([r, r * u, r * v, p / (G - 1) + 0.5 * r * (u * u + v * v)]))
+
+
+def batched(fvm, plain_rate=None, count=64, steps=200, seed=2024):
+    Us = fvm.problems.batched_instances(count=count, seed=seed)
+    shapes = [U.shape[:2] for U in Us]
+    cells = sum(ny * nx for ny, nx in shapes)
+    prm = [dict(t_end=1e30, max_steps=steps) for _ in Us]
+    b = fvm.Batch(shapes, prm)
+    for i, U in enumerate(Us):
+        b.upload(i, U)
+    tuned = b.autotune(steps_per_trial=3)
+    for i, U in enumerate(Us):
+        b.upload(i, U)
+    b.begin()
+    ms = b.run(steps)
+    done = [b.status(i)[0] for i in range(count)]
+    b.close()
+    rate = cells * steps / (ms / 1000.0)
+    # the same meshes one after another, each on its own resident solver
+    seq_ms = 0.0
+    for U in Us:
+        s = fvm.Solver(U.shape[1], U.shape[0], t_end=1e30, max_steps=steps)
+        s.upload(U)
+        n, _, m = s.run()
+        seq_ms += m
+        s.close()
+    seq_rate = cells * steps / (seq_ms / 1000.0)
+    return {"instances": count, "steps": steps, "cells": cells, "ms": ms,
+            "cell_updates_per_s": rate, "sequential_ms": seq_ms,
+            "sequential_cell_updates_per_s": seq_rate, "speedup_vs_sequential": rate / seq_rate,
+            "all_instances_stepped": all(d == steps for d in done), "autotune_ms_per_step": tuned}
+
+
+def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=10):
+    import importlib
+    import torch
+    adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
+    C0 = _cons(fvm.problems.fourier_perturbed(n, n, seed=1))
+    Ct = _cons(fvm.problems.fourier_perturbed(n, n, seed=2))
+    adj = adjmod.Adjoint([(n, n)], [{}], nsteps=steps, checkpoint_slots=slots)
+    target = adj.forward([torch.from_numpy(Ct).cuda()])[0][0].clone()
+    x = torch.from_numpy(C0).cuda().requires_grad_(True)
+    t0 = time.time()
+    UN = adjmod.FVMSolve.apply(adj, x)
+    loss = 0.5 * ((UN[0] - target) ** 2).sum()
+    loss.backward()
+    wall = time.time() - t0
+    st = adj.stats()
+    g = x.grad.cpu().numpy()
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(fd_cells):
+        k, j, i = rng.integers(0, 4), rng.integers(0, n), rng.integers(0, n)
+        h = 1e-6
+        Cp, Cm = C0.copy(), C0.copy()
+        Cp[k, j, i] += h
+        Cm[k, j, i] -= h
+        a = adj.forward([torch.from_numpy(Cp).cuda()])[0][0]
+        b = adj.forward([torch.from_numpy(Cm).cuda()])[0][0]
+        fd = float((0.5 * (a - b) * (a + b - 2 * target)).sum()) / (2 * h)
+        worst = max(worst, abs(fd - g[k, j, i]) / max(abs(fd), 1e-6 * np.abs(g).max()))
+    adj.close()
+    return {"mesh": [n, n], "steps": steps, "checkpoint_slots": slots, "loss": float(loss),
+            "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
+            "forward_step_ms": st["forward_step_ms"], "reverse_step_ms": st["reverse_step_ms"],
+            "reverse_over_forward_step": st["reverse_step_ms"] / st["forward_step_ms"],
+            "gradient_over_forward_total": (st["forward_ms"] + st["backward_ms"]) / st["forward_ms"],
+            "recompute_steps": st["recompute_steps"], "schedule_cost": st["schedule_cost"],
+            "fd_cells": fd_cells, "fd_max_rel_dev": worst, "wall_s": wall}
+
+
+def batched_adjoint(fvm, count=32, steps=300, slots=24, seed=77):
+    import importlib
+    import torch
+    adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
+    rng = fvm.problems.SplitMix64(seed)
+    shapes = [(rng.randint(256, 768), rng.randint(256, 768)) for _ in range(count)]
+    Cs = [torch.from_numpy(_cons(fvm.problems.fourier_perturbed(nx, ny, seed=i + 1))).cuda()
+          for i, (ny, nx) in enumerate(shapes)]
+    adj = adjmod.Adjoint(shapes, [{}] * count, nsteps=steps, checkpoint_slots=slots)
+    UNs = adj.forward(Cs)
+    gNs = [torch.ones_like(u) for u in UNs]
+    adj.backward(gNs)
+    st = adj.stats()
+    cells = sum(a * b for a, b in shapes)
+    adj.close()
+    return {"instances": count, "steps": steps, "cells": cells, "checkpoint_slots": slots,
+            "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
+            "forward_cell_updates_per_s": cells * steps / (st["forward_ms"] / 1000.0),
+            "reverse_over_forward_step": st["reverse_step_ms"] / st["forward_step_ms"],
+            "recompute_steps": st["recompute_steps"]}
+
+
+def peak(fvm, n=8192, steps=20, warmup=3):
+    U0 = fvm.problems.initialize_4shocks(n)
+    s = fvm.Solver(n, n, t_end=1e30)
+    s.upload(U0)
+    del U0
+    s.begin()
+    s.advance(warmup)
+    s.sync()
+    ms = s.time_steps(steps)
+    k_avg, k_min = s.time_kernel(5)
+    s.close()
+    rate = n * n * steps / (ms / 1000.0)
+    return {"mesh": [n, n], "steps": steps, "ms": ms, "cell_updates_per_s": rate,
+            "kernel_avg_ms": k_avg,
+            "roofline_frac": BYTES_PER_CELL_UPDATE * n * n / (k_avg / 1000.0) / 1e9 / HBM_PEAK_GBS}

@@ -252,8 +252,20 @@ FVM_HD int grp<true>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const G
     return br;
 }
 
-// Reciprocal for the fast path (correctly rounded division by default).
-FVM_HD double rcp(double x) { return 1.0 / x; }
+// Reciprocal of the fast path.  On the device: v_rcp_f64 refined by two
+// Newton steps r <- r + r (1 - x r) (within an ulp or two of 1/x for the
+// positive normal arguments it receives); on the host: IEEE division.
+FVM_HD double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
 
 template <>
 FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Gas &G,
@@ -285,25 +297,22 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
         cl = fma(rl * kappa, tmax(0.0, fma(pr - pl, icr, du)), Sl);
         icl = rcp(cl);
     }
-    if (ul - cl * irl >= 0.0) {
-        const double u = ul;
-        W = IfState{rl, pl, ul, fl.ut, el};
-        Wt.r = -fma(rl, sl.un, u * sl.r);
-        Wt.un = -fma(u, sl.un, sl.p * irl);
-        Wt.e = -fma(u, elx, pl * irl * sl.un);
-        Wt.ut = -u * sl.ut;
-        Wt.p = -fma(u, sl.p, cl * cl * irl * sl.un);
-        return 0;
-    }
-    if (ur + cr * irr <= 0.0) {
-        const double u = ur;
-        W = IfState{rr, pr, ur, fr.ut, er};
-        Wt.r = -fma(rr, sr.un, u * sr.r);
-        Wt.un = -fma(u, sr.un, sr.p * irr);
-        Wt.e = -fma(u, erx, pr * irr * sr.un);
-        Wt.ut = -u * sr.ut;
-        Wt.p = -fma(u, sr.p, cr * cr * irr * sr.un);
-        return 1;
+    const bool supL = ul - cl * irl >= 0.0;
+    const bool supR = !supL && ur + cr * irr <= 0.0;
+    if (supL || supR) {
+        // upwind state with the primitive Euler time derivatives; the two
+        // supersonic branches differ only in which side K is upwind
+        const double rK = supL ? rl : rr, pK = supL ? pl : pr, u = supL ? ul : ur;
+        const double irK = supL ? irl : irr, cK = supL ? cl : cr;
+        const double eK = supL ? el : er, eKx = supL ? elx : erx;
+        const Q &sK = supL ? sl : sr;
+        W = IfState{rK, pK, u, supL ? fl.ut : fr.ut, eK};
+        Wt.r = -fma(rK, sK.un, u * sK.r);
+        Wt.un = -fma(u, sK.un, sK.p * irK);
+        Wt.e = -fma(u, eKx, pK * irK * sK.un);
+        Wt.ut = -u * sK.ut;
+        Wt.p = -fma(u, sK.p, cK * cK * irK * sK.un);
+        return supL ? 0 : 1;
     }
     const double ics = rcp(cl + cr);
     const double um = (fma(ul, cl, fma(ur, cr, pl - pr))) * ics;
@@ -314,44 +323,37 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double dl = cl * irl * fma(-2.0 * cl, sl.un, fma(-ul, clx, -2.0 * sl.p));
     const double dr = cr * irr * fma(2.0 * cr, sr.un, fma(ur, crx, -2.0 * sr.p));
     const double Dp = fma(cr, dl, -cl * dr) * 0.5 * ics;
-    if (um <= 0.0) {
-        const double den = (ur - um) + cr * irr;
-        const double rho = cr * rcp(den);
-        const double irho = den * icr;
-        const double e = fma((pm - pr) * (pm + pr), 0.5 * icr * icr, er);
-        const double r3 = rho * rho * rho;
-        const double icr2 = icr * icr;
-        const double t1 = fma(cr * irr * irr, cr * sr.r, fma(-cr, sr.un, 1.5 * (um - ur) * crx));
-        const double rho_t = fma(-um * r3 * irr * icr2, t1, r3 * icr2 * icr * fma(cr, irho, um) * Dp);
-        const double urho = um * rho;
-        const double A = cl * irl * fma(urho, icr, 1.0) * kl;
-        const double B = cr * irr * fma(-urho, icl, 1.0) * kr;
-        const double p_t = ics * (cr * A - cl * B);
-        const double u_t = ics * fma(cr * irr * fma(-urho * cl, icr2, 1.0), kr, A);
-        const double e_t = fma(-urho * irr, fma(2.0 * (er - e) * icr, crx, fma(-pr * icr2, sr.p, erx)),
-                               pm * icr2 * p_t);
-        W = IfState{rho, pm, um, fr.ut, e};
-        Wt = IfState{rho_t, p_t, u_t, -um * (rho * irr * sr.ut), e_t};
-        return 2;
-    }
-    const double den = (um - ul) + cl * irl;
-    const double rho = cl * rcp(den);
-    const double irho = den * icl;
-    const double e = fma((pm - pl) * (pm + pl), 0.5 * icl * icl, el);
+    // The two star sub-cases (u* <= 0: right state K = r, s = -1; u* > 0:
+    // left state K = l, s = +1) are mirror images of each other except for
+    // the 1.5 (u* - u_K) c_Kx term, which keeps its sign in both.
+    const bool right = um <= 0.0;
+    const double s = right ? -1.0 : 1.0;
+    const double cK = right ? cr : cl, icK = right ? icr : icl;
+    const double rK = right ? rr : rl, irK = right ? irr : irl;
+    const double uK = right ? ur : ul, pK = right ? pr : pl;
+    const double eK = right ? er : el, eKx = right ? erx : elx, cKx = right ? crx : clx;
+    const Q &sK = right ? sr : sl;
+    (void)rK;
+    const double den = fma(s, um - uK, cK * irK);
+    const double rho = cK * rcp(den);
+    const double irho = den * icK;
+    const double icK2 = icK * icK;
+    const double e = fma((pm - pK) * (pm + pK), 0.5 * icK2, eK);
     const double r3 = rho * rho * rho;
-    const double icl2 = icl * icl;
-    const double t1 = fma(cl * irl * irl, cl * sl.r, fma(cl, sl.un, 1.5 * (um - ul) * clx));
-    const double rho_t = fma(-um * r3 * irl * icl2, t1, -r3 * icl2 * icl * fma(-cl, irho, um) * Dp);
+    const double t1 = fma(cK * irK * irK, cK * sK.r, fma(s * cK, sK.un, 1.5 * (um - uK) * cKx));
+    const double rho_t = fma(-um * r3 * irK * icK2, t1, r3 * icK2 * icK * fma(cK, irho, -s * um) * Dp);
     const double urho = um * rho;
     const double A = cl * irl * fma(urho, icr, 1.0) * kl;
     const double B = cr * irr * fma(-urho, icl, 1.0) * kr;
     const double p_t = ics * (cr * A - cl * B);
-    const double u_t = ics * fma(cl * irl * fma(urho * cr, icl2, 1.0), kl, B);
-    const double e_t = fma(-urho * irl, fma(2.0 * (el - e) * icl, clx, fma(-pl * icl2, sl.p, elx)),
-                           pm * icl2 * p_t);
-    W = IfState{rho, pm, um, fl.ut, e};
-    Wt = IfState{rho_t, p_t, u_t, -um * (rho * irl * sl.ut), e_t};
-    return 3;
+    const double a1 = right ? icr : cr * icl * icl;
+    const double a2 = right ? cl * icr * icr : icl;
+    const double u_t = ics * fma(cl * irl * fma(urho, a1, 1.0), kl, cr * irr * fma(-urho, a2, 1.0) * kr);
+    const double e_t = fma(-urho * irK, fma(2.0 * (eK - e) * icK, cKx, fma(-pK * icK2, sK.p, eKx)),
+                           pm * icK2 * p_t);
+    W = IfState{rho, pm, um, right ? fr.ut : fl.ut, e};
+    Wt = IfState{rho_t, p_t, u_t, -um * (rho * irK * sK.ut), e_t};
+    return right ? 2 : 3;
 }
 
 // ---------------------------------------------------------------- flux
@@ -419,7 +421,7 @@ FVM_HD double speed(double rho, double p, double u, double v, const Gas &G) {
         double velo = sqrt(u * u + v * v), c = sqrt(G.gamma * p / rho);
         return fabs(velo) + c;
     }
-    return sqrt(fma(u, u, v * v)) + sqrt(G.gamma * p / rho);
+    return sqrt(fma(u, u, v * v)) + sqrt(G.gamma * p * rcp(rho));
 }
 
 }  // namespace fvm2d

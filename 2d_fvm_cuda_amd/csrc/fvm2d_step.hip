@@ -89,7 +89,7 @@ __device__ __forceinline__ void store4(double *base, long long plane, long long 
 // Write every ghost cell whose fill source (or, for constants, whose
 // clamped owner) is interior cell (j, c) holding value (r, p, u, v).
 template <bool Exact>
-__device__ void emit_ghosts(const InstDesc &d, double *dst, int j, int c, double r, double p,
+__device__ __noinline__ void emit_ghosts(const InstDesc &d, double *dst, int j, int c, double r, double p,
                             double u, double v, double t_new, unsigned long long &mx) {
     const int nx = d.nx, ny = d.ny;
     const int rows[5] = {j, 0, 1, ny - 2, ny - 1};

@@ -59,7 +59,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--extras", action="store_true", help="also time the batched and adjoint paths")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -131,11 +132,15 @@ def main():
     if tuned_ms is not None:
         out["config"]["autotune_best_ms_per_step"] = tuned_ms
 
-    if args.extras:
-        from importlib import import_module
-        extras = import_module("2d_fvm_cuda_amd.benchmarks")
-        out["batched"] = extras.batched(fvm, plain_rate=None)
-        out["adjoint"] = extras.adjoint(fvm)
+    if not args.no_extras:
+        extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
+        for key, fn in (("batched", extras.batched), ("adjoint", extras.adjoint),
+                        ("batched_adjoint", extras.batched_adjoint), ("peak_8192", extras.peak)):
+            try:
+                out[key] = fn(fvm)
+            except Exception as e:  # a failing leg is reported, not fatal to the headline
+                out[key] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"bench: {key} done", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
