@@ -2,12 +2,16 @@
 ``torch.autograd.Function``.
 
     adj = Adjoint([(ny, nx)], [params], nsteps=400, checkpoint_slots=32)
-    U_N = FVMSolve.apply(adj, U0)          # U0: cuda float64 [4, ny, nx]
+    U_N = FVMSolve.apply(adj, U0)          # U0: float64 CPU tensor [4, ny, nx]
     loss = 0.5 * ((U_N[0] - target) ** 2).sum()
     loss.backward()                        # U0.grad = dL/dU0
 
-States are conserved planes (rho, rho u, rho v, E) in float64 on the GPU,
-row 0 = north, ghost layers included.  The forward runs ``nsteps`` steps with
+States are conserved planes (rho, rho u, rho v, E) in float64, row 0 =
+north, ghost layers included.  Tensors cross the boundary in host memory:
+the library stages them into HBM and runs the whole forward and reverse
+sweeps on the GPU.  (PyTorch's wheels bundle their own HIP runtime, which
+cannot initialise a device that libfvm2d's runtime already holds in the same
+process, so torch's GPU memory is not used here.)  The forward runs ``nsteps`` steps with
 dt = CFL h / s_max each (no t_end clamp) and stores binomial checkpoints; the
 backward recomputes from them and runs the reverse of every step
 (fvm2d_adj_backward).  Several instances (a batch) advance together:
@@ -33,8 +37,8 @@ class AdjStats(ctypes.Structure):
     ]
 
 
-EXPORTS = ["fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
-           "fvm2d_adj_set_launch", "fvm2d_adj_destroy"]
+EXPORTS = ["fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_forward_host",
+           "fvm2d_adj_backward_host", "fvm2d_adj_get_stats", "fvm2d_adj_set_launch", "fvm2d_adj_destroy"]
 
 
 def bind(L):
@@ -48,6 +52,10 @@ def bind(L):
     L.fvm2d_adj_forward.restype = ctypes.c_int
     L.fvm2d_adj_backward.argtypes = [_vp, pp, pp]
     L.fvm2d_adj_backward.restype = ctypes.c_int
+    L.fvm2d_adj_forward_host.argtypes = [_vp, pp, pp]
+    L.fvm2d_adj_forward_host.restype = ctypes.c_int
+    L.fvm2d_adj_backward_host.argtypes = [_vp, pp, pp]
+    L.fvm2d_adj_backward_host.restype = ctypes.c_int
     L.fvm2d_adj_get_stats.argtypes = [_vp, ctypes.POINTER(AdjStats)]
     L.fvm2d_adj_get_stats.restype = ctypes.c_int
     L.fvm2d_adj_set_launch.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -56,11 +64,21 @@ def bind(L):
     L.fvm2d_adj_destroy.restype = ctypes.c_int
 
 
-def _ptrs(tensors):
-    arr = (_vp * len(tensors))()
-    for i, t in enumerate(tensors):
-        arr[i] = t.data_ptr()
+def _ptrs(arrays):
+    arr = (_vp * len(arrays))()
+    for i, a in enumerate(arrays):
+        arr[i] = a.ctypes.data
     return arr
+
+
+def _as_host(x):
+    """numpy float64 C-contiguous view/copy of a numpy array or CPU tensor."""
+    import numpy as np
+    if hasattr(x, "detach"):
+        if x.device.type != "cpu":
+            raise ValueError("pass CPU tensors: the library stages them into HBM itself")
+        x = x.detach().numpy()
+    return np.ascontiguousarray(x, dtype=np.float64)
 
 
 class Adjoint:
@@ -92,33 +110,31 @@ class Adjoint:
         except Exception:
             pass
 
-    def _check_states(self, ts):
-        import torch
-        assert len(ts) == self.n
-        for t, (ny, nx) in zip(ts, self.shapes):
-            if t.dtype != torch.float64 or not t.is_cuda or tuple(t.shape) != (4, ny, nx):
-                raise ValueError(f"expected a cuda float64 tensor of shape (4, {ny}, {nx})")
-            if not t.is_contiguous():
-                raise ValueError("state tensors must be contiguous")
+    def _arrays(self, xs):
+        assert len(xs) == self.n, "one state per instance"
+        out = []
+        for x, (ny, nx) in zip(xs, self.shapes):
+            a = _as_host(x)
+            if a.shape != (4, ny, nx):
+                raise ValueError(f"expected shape (4, {ny}, {nx}), got {a.shape}")
+            out.append(a)
+        return out
+
+    def _call(self, fn, xs, what):
+        import numpy as np
+        from . import check
+        ins = self._arrays(xs)
+        outs = [np.empty_like(a) for a in ins]
+        check(fn(self.h, _ptrs(ins), _ptrs(outs)), what)
+        return outs
 
     def forward(self, U0s):
-        import torch
-        from . import check
-        self._check_states(U0s)
-        outs = [torch.empty_like(u) for u in U0s]
-        torch.cuda.synchronize()
-        check(self._lib.fvm2d_adj_forward(self.h, _ptrs(U0s), _ptrs(outs)), "fvm2d_adj_forward")
-        return outs
+        """Conserved planes U_N (numpy) of every instance after nsteps steps."""
+        return self._call(self._lib.fvm2d_adj_forward_host, U0s, "fvm2d_adj_forward_host")
 
     def backward(self, gNs):
-        import torch
-        from . import check
-        gNs = [g.contiguous() for g in gNs]
-        self._check_states(gNs)
-        outs = [torch.empty_like(g) for g in gNs]
-        torch.cuda.synchronize()
-        check(self._lib.fvm2d_adj_backward(self.h, _ptrs(gNs), _ptrs(outs)), "fvm2d_adj_backward")
-        return outs
+        """dL/dU_0 (numpy) of every instance given dL/dU_N, for the last forward."""
+        return self._call(self._lib.fvm2d_adj_backward_host, gNs, "fvm2d_adj_backward_host")
 
     def stats(self) -> dict:
         from . import check
@@ -142,13 +158,13 @@ def _function():
         @staticmethod
         def forward(ctx, adj, *U0s):
             ctx.adj = adj
-            outs = adj.forward([u.detach().contiguous() for u in U0s])
+            outs = [torch.from_numpy(o) for o in adj.forward(list(U0s))]
             return outs[0] if len(outs) == 1 else tuple(outs)
 
         @staticmethod
         def backward(ctx, *gNs):
-            gs = ctx.adj.backward(list(gNs))
-            return (None, *gs)
+            gs = ctx.adj.backward([g.contiguous() for g in gNs])
+            return (None, *[torch.from_numpy(g) for g in gs])
 
     return FVMSolve
 

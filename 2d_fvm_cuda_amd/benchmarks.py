@@ -65,15 +65,15 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=10):
     C0 = _cons(fvm.problems.fourier_perturbed(n, n, seed=1))
     Ct = _cons(fvm.problems.fourier_perturbed(n, n, seed=2))
     adj = adjmod.Adjoint([(n, n)], [{}], nsteps=steps, checkpoint_slots=slots)
-    target = adj.forward([torch.from_numpy(Ct).cuda()])[0][0].clone()
-    x = torch.from_numpy(C0).cuda().requires_grad_(True)
+    target = adj.forward([Ct])[0][0]
+    x = torch.from_numpy(C0.copy()).requires_grad_(True)
     t0 = time.time()
     UN = adjmod.FVMSolve.apply(adj, x)
-    loss = 0.5 * ((UN[0] - target) ** 2).sum()
+    loss = 0.5 * ((UN[0] - torch.from_numpy(target)) ** 2).sum()
     loss.backward()
     wall = time.time() - t0
     st = adj.stats()
-    g = x.grad.cpu().numpy()
+    g = x.grad.numpy()
     rng = np.random.default_rng(11)
     worst = 0.0
     for _ in range(fd_cells):
@@ -82,8 +82,8 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=10):
         Cp, Cm = C0.copy(), C0.copy()
         Cp[k, j, i] += h
         Cm[k, j, i] -= h
-        a = adj.forward([torch.from_numpy(Cp).cuda()])[0][0]
-        b = adj.forward([torch.from_numpy(Cm).cuda()])[0][0]
+        a = adj.forward([Cp])[0][0]
+        b = adj.forward([Cm])[0][0]
         fd = float((0.5 * (a - b) * (a + b - 2 * target)).sum()) / (2 * h)
         worst = max(worst, abs(fd - g[k, j, i]) / max(abs(fd), 1e-6 * np.abs(g).max()))
     adj.close()
@@ -98,16 +98,13 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=10):
 
 def batched_adjoint(fvm, count=32, steps=300, slots=24, seed=77):
     import importlib
-    import torch
     adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
     rng = fvm.problems.SplitMix64(seed)
     shapes = [(rng.randint(256, 768), rng.randint(256, 768)) for _ in range(count)]
-    Cs = [torch.from_numpy(_cons(fvm.problems.fourier_perturbed(nx, ny, seed=i + 1))).cuda()
-          for i, (ny, nx) in enumerate(shapes)]
+    Cs = [_cons(fvm.problems.fourier_perturbed(nx, ny, seed=i + 1)) for i, (ny, nx) in enumerate(shapes)]
     adj = adjmod.Adjoint(shapes, [{}] * count, nsteps=steps, checkpoint_slots=slots)
     UNs = adj.forward(Cs)
-    gNs = [torch.ones_like(u) for u in UNs]
-    adj.backward(gNs)
+    adj.backward([np.ones_like(u) for u in UNs])
     st = adj.stats()
     cells = sum(a * b for a, b in shapes)
     adj.close()

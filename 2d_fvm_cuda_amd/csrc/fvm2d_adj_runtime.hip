@@ -34,6 +34,7 @@ struct AdjEngine {
     int4 *d_tfwd = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
     int nfwd = 0, nr2 = 0, nr3a = 0, nr3b = 0;
     std::vector<void *> allocs;
+    std::vector<double *> stage_in, stage_out;  // host-pointer API staging
     // schedule
     std::vector<std::vector<long long>> cost;
     std::vector<std::vector<int>> split;
@@ -451,6 +452,42 @@ int fvm2d_adj_forward(fvm2d_adj *a, const double *const *d_U0_cons, double *cons
 int fvm2d_adj_backward(fvm2d_adj *a, const double *const *d_gN_cons, double *const *d_g0_cons) {
     if (!a || !d_gN_cons || !d_g0_cons) return fail(FVM2D_ERR_ARG, "null argument");
     return a->eng.backward(d_gN_cons, d_g0_cons);
+}
+
+// Host-pointer variants: stage through per-instance device buffers.
+static int adj_host_call(fvm2d_adj *a, const double *const *in, double *const *out, bool fwd) {
+    if (!a || !in || !out) return fail(FVM2D_ERR_ARG, "null argument");
+    AdjEngine &E = a->eng;
+    if (E.stage_in.empty()) {
+        E.stage_in.resize(E.n);
+        E.stage_out.resize(E.n);
+        for (int i = 0; i < E.n; i++) {
+            const size_t cnt = 4 * (size_t)E.nx[i] * E.ny[i];
+            int rc = E.dalloc(&E.stage_in[i], cnt);
+            if (!rc) rc = E.dalloc(&E.stage_out[i], cnt);
+            if (rc) return rc;
+        }
+    }
+    for (int i = 0; i < E.n; i++) {
+        if (!in[i] || !out[i]) return fail(FVM2D_ERR_ARG, "null host pointer");
+        HIPCHK(hipMemcpy(E.stage_in[i], in[i], 4 * (size_t)E.nx[i] * E.ny[i] * sizeof(double),
+                         hipMemcpyHostToDevice));
+    }
+    const int rc = fwd ? E.forward(E.stage_in.data(), E.stage_out.data())
+                       : E.backward(E.stage_in.data(), E.stage_out.data());
+    if (rc) return rc;
+    for (int i = 0; i < E.n; i++)
+        HIPCHK(hipMemcpy(out[i], E.stage_out[i], 4 * (size_t)E.nx[i] * E.ny[i] * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return FVM2D_OK;
+}
+
+int fvm2d_adj_forward_host(fvm2d_adj *a, const double *const *U0_cons, double *const *UN_cons) {
+    return adj_host_call(a, U0_cons, UN_cons, true);
+}
+
+int fvm2d_adj_backward_host(fvm2d_adj *a, const double *const *gN_cons, double *const *g0_cons) {
+    return adj_host_call(a, gN_cons, g0_cons, false);
 }
 
 int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *st) {

@@ -63,13 +63,13 @@ __device__ __forceinline__ long long off_of(const LineMap &m) { return m.base + 
 __device__ __forceinline__ Q load_line(const double *a, long long plane, const LineMap &m) {
     const long long o = off_of(m);
     const Q q{a[o], a[plane + o], a[2 * plane + o], a[3 * plane + o]};
-    return m.ydir ? swap_uv(q) : q;
+    return pick(m.ydir, swap_uv(q), q);
 }
 
 __device__ __forceinline__ void store_line(double *a, long long plane, const LineMap &m, const Q &q,
                                            bool accumulate) {
     const long long o = off_of(m);
-    const Q x = m.ydir ? swap_uv(q) : q;
+    const Q x = pick(m.ydir, swap_uv(q), q);
     if (accumulate) {
         a[o] += x.r;
         a[plane + o] += x.p;
@@ -83,8 +83,17 @@ __device__ __forceinline__ void store_line(double *a, long long plane, const Lin
     }
 }
 
-__device__ __forceinline__ const Line &line_of(const AdjDesc &d, const LineMap &m) {
-    return m.ydir ? d.ly : d.lx;
+// Geometry of the task's line, selected field by field (a selected struct
+// reference would be materialised in private memory).
+__device__ __forceinline__ Line line_of(const AdjDesc &d, const LineMap &m) {
+    Line L;
+    L.h = m.ydir ? d.ly.h : d.lx.h;
+    L.ih = m.ydir ? d.ly.ih : d.lx.ih;
+    L.ih_a = m.ydir ? d.ly.ih_a : d.lx.ih_a;
+    L.ih_c = m.ydir ? d.ly.ih_c : d.lx.ih_c;
+    L.hh = m.ydir ? d.ly.hh : d.lx.hh;
+    L.alpha = m.ydir ? d.ly.alpha : d.lx.alpha;
+    return L;
 }
 
 __device__ __forceinline__ unsigned long long bits_of(double x) {
@@ -108,7 +117,9 @@ __device__ Q lamC_at(const AdjDesc &d, const double *lam, int j, int c) {
     if (j == 2 || j == ny - 4 || j == ny - 3 || c == 2 || c == nx - 3) {
         const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
         const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
+#pragma unroll
         for (int a = 0; a < 5; a++) {
+#pragma unroll
             for (int b = 0; b < 5; b++) {
                 if (a == 0 && b == 0) continue;
                 const int gj = rows[a], gi = cols[b];
@@ -137,13 +148,13 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
     const int4 t = tasks[tk];
     AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
     const LineMap m = line_map(d, t, lane, kFwdHalo, kFwdOwn);
-    const Line &L = line_of(d, m);
+    const Line L = line_of(d, m);
     const double dt = d.ctl->dt;
     const Q q = load_line(d.U, d.plane, m);
     const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
     const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
     const bool upd = m.k >= 2 && m.k <= m.n - 3;
-    if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, upd ? qo : q, false);
+    if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
 }
 
 // s_max over every cell of U_n with the smallest linear index attaining it;
@@ -230,7 +241,7 @@ __global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ de
     const int4 t = tasks[tk];
     AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
     const LineMap m = line_map(d, t, lane, kAdjHalo, kAdjOwn);
-    const Line &L = line_of(d, m);
+    const Line L = line_of(d, m);
     const Gas &G = d.gas;
     const double dt = d.ctl->dt;
     const double lam = dt / L.h;
@@ -244,7 +255,7 @@ __global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ de
         const int j = (int)(o / d.pitch), c = (int)(o % d.pitch);
         const Q s = lamC_at(d, d.lam_in, j, c);
         const Q h{0.5 * s.r, 0.5 * s.p, 0.5 * s.un, 0.5 * s.ut};
-        qbo = m.ydir ? swap_uv(h) : h;
+        qbo = pick(m.ydir, swap_uv(h), h);
     } else {
         qin = d.U;
         qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
@@ -298,7 +309,7 @@ __global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ de
     Q pL, pC, pR;
     slope_adj(qL, q, qR, L, st, pL, pC, pR);
     const Q fromR = shl(pL), fromL = shr(pR);
-    const Q cb = upd ? cons_adj(q, Ub, G) : qbo;
+    const Q cb = pick(upd, cons_adj(q, Ub, G), qbo);
     qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r, qb.p + pC.p + fromR.p + fromL.p + cb.p,
            qb.un + pC.un + fromR.un + fromL.un + cb.un, qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut};
 

@@ -76,6 +76,11 @@ FVM_HD Line make_line(double h, double alpha) {
     return L;
 }
 
+// Field-wise select (a selected struct reference would live in private memory).
+FVM_HD Q pick(bool c, const Q &a, const Q &b) {
+    return Q{c ? a.r : b.r, c ? a.p : b.p, c ? a.un : b.un, c ? a.ut : b.ut};
+}
+
 FVM_HD double tmin(double a, double b) { return b < a ? b : a; }
 FVM_HD double tmax(double a, double b) { return a < b ? b : a; }
 
@@ -305,7 +310,7 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
         const double rK = supL ? rl : rr, pK = supL ? pl : pr, u = supL ? ul : ur;
         const double irK = supL ? irl : irr, cK = supL ? cl : cr;
         const double eK = supL ? el : er, eKx = supL ? elx : erx;
-        const Q &sK = supL ? sl : sr;
+        const Q sK = pick(supL, sl, sr);
         W = IfState{rK, pK, u, supL ? fl.ut : fr.ut, eK};
         Wt.r = -fma(rK, sK.un, u * sK.r);
         Wt.un = -fma(u, sK.un, sK.p * irK);
@@ -332,7 +337,7 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double rK = right ? rr : rl, irK = right ? irr : irl;
     const double uK = right ? ur : ul, pK = right ? pr : pl;
     const double eK = right ? er : el, eKx = right ? erx : elx, cKx = right ? crx : clx;
-    const Q &sK = right ? sr : sl;
+    const Q sK = pick(right, sr, sl);
     (void)rK;
     const double den = fma(s, um - uK, cK * irK);
     const double rho = cK * rcp(den);

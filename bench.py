@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--n", type=int, default=2000)
     ap.add_argument("--mode", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")

@@ -157,6 +157,10 @@ int fvm2d_adj_create(int n, const int *nx, const int *ny, const fvm2d_params *p,
                      int checkpoint_slots, fvm2d_adj **out);
 int fvm2d_adj_forward(fvm2d_adj *a, const double *const *d_U0_cons, double *const *d_UN_cons);
 int fvm2d_adj_backward(fvm2d_adj *a, const double *const *d_gN_cons, double *const *d_g0_cons);
+/* Same with host pointers: the library stages the planes through its own
+ * device buffers (the timings in fvm2d_adj_stats exclude the copies). */
+int fvm2d_adj_forward_host(fvm2d_adj *a, const double *const *U0_cons, double *const *UN_cons);
+int fvm2d_adj_backward_host(fvm2d_adj *a, const double *const *gN_cons, double *const *g0_cons);
 int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *out);
 int fvm2d_adj_set_launch(fvm2d_adj *a, int strip_rows, int waves_per_block, int persistent);
 int fvm2d_adj_destroy(fvm2d_adj *a);
