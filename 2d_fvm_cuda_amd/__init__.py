@@ -67,7 +67,7 @@ EXPORTS = [
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
-    "fvm2d_batch_autotune", "fvm2d_batch_destroy",
+    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe",
     "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
     "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
@@ -118,6 +118,7 @@ def lib():
         "fvm2d_batch_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "fvm2d_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "fvm2d_batch_destroy": (ctypes.c_int, [_vp]),
+        "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -142,6 +143,14 @@ def check(rc: int, what: str = "") -> None:
 
 def device_available() -> bool:
     return bool(lib().fvm2d_device_available())
+
+
+def hbm_copy_probe(n_doubles: int, reps: int) -> float:
+    """Mean ms of one plain device copy of n_doubles FP64 values (the HBM
+    counter calibration kernel)."""
+    ms = ctypes.c_double(0.0)
+    check(lib().fvm2d_hbm_copy_probe(int(n_doubles), int(reps), ctypes.byref(ms)), "fvm2d_hbm_copy_probe")
+    return ms.value
 
 
 def make_params(width=1.0, height=1.0, t_end=0.0, cfl=0.5, alpha=1.9, gamma=1.4,

@@ -286,36 +286,38 @@ struct AdjEngine {
     // Reverse steps a .. a+len-1; the state U_a is in slot sa.
     int reverse(int a, int len, int c, int sa) {
         if (len == 1) return reverse_step(sa);
-        if (c == 0 || take_slot() < 0) {
-            for (int k = len - 1; k >= 1; k--) {
-                int rc = advance(sa, k, slot_t[sa], nullptr);
+        if (c > 0) {
+            const int m = split[len][std::min(c, (int)split[len].size() - 1)];
+            int s = -1;
+            auto it = stored.find(a + m);
+            if (it != stored.end()) {
+                s = it->second;  // placed by the forward call's spine
+            } else if ((s = take_slot()) >= 0) {
+                std::vector<double> t;
+                int rc = advance(sa, m, slot_t[sa], &t);
                 if (rc) return rc;
-                rc = reverse_step(-1);
+                rc = copy_state(-1, s);
                 if (rc) return rc;
+                slot_t[s] = t;
+                slot_step[s] = a + m;
+                stored[a + m] = s;
             }
-            return reverse_step(sa);
+            if (s >= 0) {
+                int rc = reverse(a + m, len - m, c - 1, s);
+                if (rc) return rc;
+                stored.erase(a + m);
+                slot_step[s] = -1;
+                return reverse(a, m, c, sa);
+            }
         }
-        const int m = split[len][std::min(c, (int)split[len].size() - 1)];
-        int s;
-        auto it = stored.find(a + m);
-        if (it != stored.end()) {
-            s = it->second;
-        } else {
-            s = take_slot();
-            std::vector<double> t;
-            int rc = advance(sa, m, slot_t[sa], &t);
+        // no checkpoint to spare: recompute every state from U_a
+        for (int k = len - 1; k >= 1; k--) {
+            int rc = advance(sa, k, slot_t[sa], nullptr);
             if (rc) return rc;
-            rc = copy_state(-1, s);
+            rc = reverse_step(-1);
             if (rc) return rc;
-            slot_t[s] = t;
-            slot_step[s] = a + m;
-            stored[a + m] = s;
         }
-        int rc = reverse(a + m, len - m, c - 1, s);
-        if (rc) return rc;
-        stored.erase(a + m);
-        slot_step[s] = -1;
-        return reverse(a, m, c, sa);
+        return reverse_step(sa);
     }
 
     int forward(const double *const *U0_cons, double *const *UN_cons) {
@@ -388,6 +390,7 @@ struct AdjEngine {
         recompute_steps = 0;
         reverse_steps = 0;
         rev_ms_sum = 0.0;
+        HIPCHK(hipMemsetAsync(d_actl, 0, n * sizeof(AdjCtl), fwd.stream));
         hipEvent_t b0, b1;
         HIPCHK(hipEventCreate(&b0));
         HIPCHK(hipEventCreate(&b1));

@@ -428,13 +428,33 @@ struct Engine {
         }
         int maxny = 0;
         for (int i = 0; i < n; i++) maxny = std::max(maxny, hd[i].ny - 4);
-        const int Rs[] = {4, 8, 12, 16, 24, 32, 48, 64};
+        // Strip heights: a fixed ladder plus, for k = 1..8 rounds of resident
+        // waves, the height that makes the wave count just fit k rounds
+        // (a launch whose last round is nearly empty wastes that round).
+        std::vector<int> Rs = {4, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64};
+        {
+            const int occ = std::max(1, step_kernel_occupancy(exact, 4));
+            const long long slots = (long long)occ * 4 * cu_count;
+            long long cols = 0;  // windows summed over instances, weighted by rows
+            long long rows_max = 0;
+            for (int i = 0; i < n; i++) {
+                cols += hd[i].nw;
+                rows_max = std::max<long long>(rows_max, hd[i].ny - 4);
+            }
+            for (int k = 1; k <= 8 && n == 1; k++) {
+                const long long strips = std::max<long long>(1, k * slots / std::max<long long>(1, cols));
+                const int r = (int)((rows_max + strips - 1) / strips);
+                if (r >= 4 && r <= 96) Rs.push_back(r);
+            }
+            std::sort(Rs.begin(), Rs.end());
+            Rs.erase(std::unique(Rs.begin(), Rs.end()), Rs.end());
+        }
         double best = std::numeric_limits<double>::infinity();
         int bR = R, bW = wpb, bP = persistent;
         for (int r : Rs) {
             if (rc) break;
             if (r > std::max(4, maxny)) continue;
-            for (int w : {1, 2, 4}) {
+            for (int w : {2, 4}) {
                 for (int pers : {0, 1}) {
                     rc = set_launch(r, w, pers);
                     if (!rc) rc = copy_states(false);

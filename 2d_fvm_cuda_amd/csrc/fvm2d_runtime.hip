@@ -305,4 +305,47 @@ int fvm2d_batch_destroy(fvm2d_batch *b) {
     return FVM2D_OK;
 }
 
+// Plain device-to-device copy of a known size, used to calibrate the HBM
+// byte counters (FETCH_SIZE / WRITE_SIZE) against the step kernel's.
+__global__ void __launch_bounds__(256) hbm_copy_kernel(const double2 *__restrict__ src,
+                                                       double2 *__restrict__ dst, long long n2) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += stride) dst[k] = src[k];
+}
+
+int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out) {
+    if (n_doubles < 2 || (n_doubles & 1) || reps < 1) return fail(FVM2D_ERR_ARG, "need an even n_doubles >= 2 and reps >= 1");
+    if (int rc0 = device_ok()) return rc0;
+    double *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, n_doubles * sizeof(double)));
+    if (hipMalloc((void **)&b, n_doubles * sizeof(double)) != hipSuccess) {
+        hipFree(a);
+        return fail(FVM2D_ERR_HIP, "hipMalloc failed");
+    }
+    hipEvent_t e0, e1;
+    int rc = FVM2D_OK;
+    if (hipMemset(a, 0, n_doubles * sizeof(double)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(FVM2D_ERR_HIP, "probe setup failed");
+    } else {
+        const long long n2 = n_doubles / 2;
+        const int grid = (int)std::min<long long>((n2 + 255) / 256, 256LL * 8 * 4);
+        hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++) hbm_copy_kernel<<<grid, 256, 0, nullptr>>>((const double2 *)a, (double2 *)b, n2);
+        hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(FVM2D_ERR_HIP, "copy probe failed");
+        } else {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (avg_ms_out) *avg_ms_out = ms / reps;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 }  // extern "C"

@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 BYTES_PER_CELL_UPDATE = 288  # 4 sweeps x (32 B read + 32 B write) + 32 B partner read
+COPY_PROBE_BYTES = 1 << 28   # 256 MiB read + 256 MiB written per copy launch
 
 
 def cpu_baseline(U0, steps: int, threads: int):
@@ -59,8 +60,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="strip_rows,waves_per_block,persistent: fixed geometry instead of autotuning")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain 256 MiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,6 +79,13 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_probe:
+        n_doubles = COPY_PROBE_BYTES // 8
+        ms = fvm.hbm_copy_probe(n_doubles, 5)
+        print(json.dumps({"copy_probe_bytes_read": COPY_PROBE_BYTES, "copy_probe_bytes_written": COPY_PROBE_BYTES,
+                          "avg_ms": ms, "GB_s": 2 * COPY_PROBE_BYTES / (ms / 1000.0) / 1e9}))
+        return
+
     N = args.n
     mode = fvm.MODE_EXACT if args.mode == "exact" else fvm.MODE_FAST
     U0 = problems.initialize_4shocks(N)
@@ -82,7 +94,10 @@ def main():
     s = fvm.Solver(N, N, **prm)
     s.upload(U0)
     tuned_ms = None
-    if not args.no_autotune:
+    if args.launch:
+        r, w, p = (int(x) for x in args.launch.split(","))
+        s.set_launch(r, w, p)
+    elif not args.no_autotune:
         tuned_ms = s.autotune(steps_per_trial=4)
     launch = s.get_launch()
 

@@ -133,6 +133,13 @@ int fvm2d_batch_set_launch(fvm2d_batch *b, int strip_rows, int waves_per_block, 
 int fvm2d_batch_autotune(fvm2d_batch *b, int steps_per_trial, double *best_ms_out);
 int fvm2d_batch_destroy(fvm2d_batch *b);
 
+/* Diagnostic with no counterpart in the reference: reps launches of a plain
+ * device-to-device copy of n_doubles FP64 values (16-byte accesses, two
+ * buffers allocated and freed by the call).  Its FETCH_SIZE / WRITE_SIZE
+ * counters calibrate the step kernel's measured HBM traffic.  avg_ms_out:
+ * mean duration of one copy. */
+int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- *
  * Hand-written reverse mode of the full split step (limiter and GRP
  * differentiated along the branch taken, dt differentiated through its
