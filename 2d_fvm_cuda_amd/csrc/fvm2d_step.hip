@@ -86,19 +86,32 @@ __device__ __forceinline__ void store4(double *base, long long plane, long long 
     base[3 * plane + off] = v;
 }
 
-// Write every ghost cell whose fill source (or, for constants, whose
-// clamped owner) is interior cell (j, c) holding value (r, p, u, v).
+// Write every ghost cell among the candidates whose fill source (or, for
+// constants, whose clamped owner) is interior cell (j, c) holding value
+// (r, p, u, v).  Candidates: rows {j, 0, 1, ny-2, ny-1} x columns
+// {c, 0, 1, nx-2, nx-1} when corner != 0 (a cell in the first or last two
+// interior rows and on the first or last interior column), else the
+// ghost rows above and below column c (j in the first or last two
+// interior rows) and the ghost columns beside row j (c on the first or
+// last interior column).  Every ghost cell of the preset has its source
+// or owner in exactly one of these sets.
 template <bool Exact>
 __device__ __noinline__ void emit_ghosts(const InstDesc &d, double *dst, int j, int c, double r, double p,
-                            double u, double v, double t_new, unsigned long long &mx) {
+                                         double u, double v, double t_new, unsigned long long &mx,
+                                         int corner) {
     const int nx = d.nx, ny = d.ny;
     const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
     const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
+    const bool top = j == 2, bot = j >= ny - 4;
+    const bool left = c == 2, right = c == nx - 3;
     for (int a = 0; a < 5; a++) {
         const int gj = rows[a];
+        if (!corner && a > 0 && !((a <= 2 && top) || (a >= 3 && bot))) continue;
         for (int b = 0; b < 5; b++) {
             const int gi = cols[b];
             if (a == 0 && b == 0) continue;
+            if (!corner && a > 0 && b > 0) continue;
+            if (!corner && a == 0 && !((b <= 2 && left) || (b >= 3 && right))) continue;
             const bool ghost = gj < 2 || gj > ny - 3 || gi < 2 || gi > nx - 3;
             if (!ghost) continue;
             const GhostSrc g = ghost_source(d.strategy, gj, gi, nx, ny, t_new);
@@ -114,6 +127,30 @@ __device__ __noinline__ void emit_ghosts(const InstDesc &d, double *dst, int j, 
             }
         }
     }
+}
+
+// Outflow preset (src/ghostCellPresets.cuh:106-145): every ghost cell is a
+// copy of the nearest interior cell, so the owner of (j, c) writes it to
+// the ghost cells beside it (c on the first/last interior column), above
+// or below it (j on the first/last interior row), and to the corner
+// blocks when both hold.
+__device__ __forceinline__ void emit_outflow(const InstDesc &d, double *dst, int j, int c, double r,
+                                             double p, double u, double v) {
+    const int nx = d.nx, ny = d.ny;
+    const long long P = d.plane, pitch = d.pitch;
+    const int nr = (j == 2 ? 2 : 0) + (j == ny - 3 ? 2 : 0);
+    const int nc = (c == 2 ? 2 : 0) + (c == nx - 3 ? 2 : 0);
+    int gr[4], gc[4];
+    int k = 0;
+    if (j == 2) gr[k++] = 0, gr[k++] = 1;
+    if (j == ny - 3) gr[k++] = ny - 2, gr[k++] = ny - 1;
+    k = 0;
+    if (c == 2) gc[k++] = 0, gc[k++] = 1;
+    if (c == nx - 3) gc[k++] = nx - 2, gc[k++] = nx - 1;
+    for (int a = 0; a < nr; a++) store4(dst, P, (long long)gr[a] * pitch + c, r, p, u, v);
+    for (int b = 0; b < nc; b++) store4(dst, P, (long long)j * pitch + gc[b], r, p, u, v);
+    for (int a = 0; a < nr; a++)
+        for (int b = 0; b < nc; b++) store4(dst, P, (long long)gr[a] * pitch + gc[b], r, p, u, v);
 }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
@@ -209,8 +246,13 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
             store4(dst, P, o, r, p, u, v);
             const unsigned long long sb = dbits(speed<Exact>(r, p, u, v, G));
             mx = sb > mx ? sb : mx;
-            if (j == 2 || j == ny - 4 || j == ny - 3 || col == 2 || col == nx - 3)
-                emit_ghosts<Exact>(d, dst, j, col, r, p, u, v, t_new, mx);
+            const bool edge_row = j == 2 || j >= ny - 4, edge_col = col == 2 || col == nx - 3;
+            if (edge_row || edge_col) {
+                if (d.strategy == kOutflow)
+                    emit_outflow(d, dst, j, col, r, p, u, v);
+                else
+                    emit_ghosts<Exact>(d, dst, j, col, r, p, u, v, t_new, mx, edge_row && edge_col);
+            }
         }
     }
 

@@ -21,16 +21,21 @@ def cons_to_prim5(C):
 
 
 def small_problem(problems, nx=22, ny=18, seed=3):
-    """Smooth seeded data: the adjoint workload's recipe plus smooth velocity
-    modes, so that no component is exactly flat (the limiter is not
-    differentiable at a flat state, where a perturbation over many cells
-    picks a direction-dependent branch)."""
+    """Smooth seeded data: the adjoint workload's recipe plus random-phase
+    velocity modes, so that no component is flat and no two neighbouring
+    values are equal by symmetry (the limiter is not differentiable where a
+    one-sided difference is zero; there a perturbation picks a
+    direction-dependent branch)."""
     U5 = problems.fourier_perturbed(nx, ny, seed)
     x = (np.arange(nx) + 0.5) / nx
     y = (np.arange(ny) + 0.5) / ny
     X, Y = np.meshgrid(x, y)
-    U5[..., 2] += 0.05 * np.sin(2 * np.pi * (X + 2 * Y) + seed)
-    U5[..., 3] += 0.05 * np.cos(2 * np.pi * (2 * X - Y) + 0.5 * seed)
+    rng = np.random.default_rng(1000 + seed)
+    for c in (2, 3):
+        for _ in range(4):
+            kx, ky = rng.integers(1, 4, size=2)
+            U5[..., c] += rng.uniform(0.01, 0.03) * np.sin(2 * np.pi * (kx * X + ky * Y) +
+                                                           rng.uniform(0, 2 * np.pi))
     return to_cons(U5)
 
 
@@ -71,27 +76,41 @@ def test_host_gradient_directional_fd(orc, problems, strategy):
 
 
 def test_host_gradient_single_cells_fd(orc, problems):
+    """Every (component, cell) of a 20 x 20 mesh over 5 steps: the gradient
+    of L = 1/2 sum (rho_N - target)^2 against the central and one-sided
+    difference quotients (h = 1e-6 scaled), the best of the three counting
+    (a quotient whose interval holds a branch switch is not the derivative
+    along the taken branch)."""
     C0 = small_problem(problems, 20, 20, seed=5)
     n = 5
     rng = np.random.default_rng(1)
     target = rng.normal(size=C0.shape[1:]) * 0.1 + C0[0]
 
     UN = orc.adj_forward(C0, n)
+    r0 = UN[0]
 
-    def dloss(Cp, Cm):
-        """L(Cp) - L(Cm) evaluated cell by cell before summing."""
-        a, b = orc.adj_forward(Cp, n)[0], orc.adj_forward(Cm, n)[0]
-        return np.sum(0.5 * (a - b) * (a + b - 2 * target))
+    def dloss(Cp):
+        """L(Cp) - L(C0) evaluated cell by cell before summing."""
+        a = orc.adj_forward(Cp, n)[0]
+        return np.sum(0.5 * (a - r0) * (a + r0 - 2 * target))
 
     gN = np.zeros_like(C0)
     gN[0] = UN[0] - target
     g0 = orc.adj_backward(C0, n, gN)
-    for _ in range(12):
-        k = rng.integers(0, 4)
-        j, i = rng.integers(0, 20), rng.integers(0, 20)
-        h = 1e-6 * max(1.0, abs(C0[k, j, i]))
-        Cp, Cm = C0.copy(), C0.copy()
-        Cp[k, j, i] += h
-        Cm[k, j, i] -= h
-        fd = dloss(Cp, Cm) / (2 * h)
-        assert g0[k, j, i] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+    gmax = np.abs(g0).max()
+    worst, bad = 0.0, 0
+    for k in range(4):
+        for j in range(20):
+            for i in range(20):
+                h = 1e-6 * max(1.0, abs(C0[k, j, i]))
+                Cp, Cm = C0.copy(), C0.copy()
+                Cp[k, j, i] += h
+                Cm[k, j, i] -= h
+                fp, fm = dloss(Cp) / h, -dloss(Cm) / h
+                a = g0[k, j, i]
+                d = min(abs(a - 0.5 * (fp + fm)), abs(a - fp), abs(a - fm)) / max(abs(a), 1e-3 * gmax)
+                worst = max(worst, d)
+                bad += d > 1e-6
+    print(f"worst best-of-three deviation {worst:.2e}, pairs above 1e-6: {bad} of 1600")
+    assert bad <= 8
+    assert worst < 1e-5
