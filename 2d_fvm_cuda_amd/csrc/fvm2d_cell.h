@@ -13,9 +13,10 @@
 //                   contraction it reproduces the host FP64 restatement bit
 //                   for bit.
 //   Exact = false : the same mathematics with reciprocals shared between
-//                   terms (8 reciprocals + 2 square roots per interface
-//                   instead of ~62 divisions + 4 square roots) and FMAs.
-//                   Results differ from Exact only by rounding.
+//                   terms (3 reciprocals + 2 reciprocal square roots per
+//                   interface instead of ~62 divisions + 4 square roots),
+//                   FMAs and a sign-test-free limiter.  Results differ from
+//                   Exact only by rounding.
 #pragma once
 
 #include <cmath>
@@ -89,28 +90,28 @@ FVM_HD double tmax(double a, double b) { return a < b ? b : a; }
 // (weight 1/2) and the one-sided differences (weight alpha).
 template <bool Exact>
 FVM_HD double limit1(double qL, double q, double qR, const Line &L) {
-    double c, l, r;
-    if (Exact) {
-        c = (qR - qL) * 0.5 * L.ih;
-        l = (q - qL) * L.ih * L.alpha;
-        r = (qR - q) * L.ih * L.alpha;
-    } else {
-        c = (qR - qL) * L.ih_c;
-        l = (q - qL) * L.ih_a;
-        r = (qR - q) * L.ih_a;
-    }
+    const double c = (qR - qL) * 0.5 * L.ih;
+    const double l = (q - qL) * L.ih * L.alpha;
+    const double r = (qR - q) * L.ih * L.alpha;
     double s13 = c * r, s23 = l * r;
     if (s13 > 0.0 && s23 > 0.0) {
-        if (Exact) {
-            double sgn = 1.0;
-            if (c <= 0.0) sgn = -1.0;
-            return sgn * tmin(fabs(c), tmin(fabs(l), fabs(r)));
-        } else {
-            // all three share the sign of c here
-            return copysign(fmin(fabs(c), fmin(fabs(l), fabs(r))), c);
-        }
+        double sgn = 1.0;
+        if (c <= 0.0) sgn = -1.0;
+        return sgn * tmin(fabs(c), tmin(fabs(l), fabs(r)));
     }
     return 0.0;
+}
+
+// Fast form of the same limiter.  With dC = qR - qL and sa = alpha carrying
+// the sign of dC, min(sa dL, sa dR, |dC|/2) is positive exactly when dL and
+// dR share the sign of dC (both one-sided differences agree), so clamping
+// it at zero replaces the two sign tests; the scale 1/h is applied last.
+template <>
+FVM_HD double limit1<false>(double qL, double q, double qR, const Line &L) {
+    const double dL = q - qL, dR = qR - q, dC = qR - qL;
+    const double sa = copysign(L.alpha, dC);
+    const double m = fmax(0.0, fmin(fmin(sa * dL, sa * dR), 0.5 * fabs(dC)));
+    return m * copysign(L.ih, dC);
 }
 
 template <bool Exact>
@@ -272,36 +273,58 @@ FVM_HD double rcp(double x) {
 #endif
 }
 
+// Reciprocal square root of the fast path (positive normal x).  On the
+// device: v_rsq_f64 refined by two Newton steps y <- y + y (1/2 - (x/2) y^2);
+// on the host: IEEE operations.
+FVM_HD double rsq(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return fma(y, fma(-hx * y, y, 0.5), y);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 template <>
 FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Gas &G,
                       IfState &W, IfState &Wt) {
     const double gamma = G.gamma, g1i = G.g1i, kappa = G.kappa;
     const double rl = fl.r, rr = fr.r, pl = fl.p, pr = fr.p;
     const double ul = fl.un, ur = fr.un;
-    const double irl = rcp(rl), irr = rcp(rr);
-    const double el = pl * irl * g1i, er = pr * irr * g1i;
+    // Lagrangian sound speeds S = rho a = sqrt(gamma p rho) from one
+    // reciprocal square root each; 1/rho = gamma p / S^2
+    const double gpl = gamma * pl, gpr = gamma * pr;
+    const double xl = gpl * rl, xr = gpr * rr;
+    const double iSl = rsq(xl), iSr = rsq(xr);
+    const double Sl = xl * iSl, Sr = xr * iSr;
+    const double irl = gpl * iSl * iSl, irr = gpr * iSr * iSr;
+    const double prl = pl * irl, prr = pr * irr;
+    const double el = prl * g1i, er = prr * g1i;
     // e_x = (p_x - p rho_x / rho) / rho / (gamma - 1)
-    const double elx = fma(-pl * irl, sl.r, sl.p) * irl * g1i;
-    const double erx = fma(-pr * irr, sr.r, sr.p) * irr * g1i;
-    // Lagrangian sound speeds S = rho a = sqrt(gamma p rho)
-    const double Sl = sqrt(gamma * pl * rl), Sr = sqrt(gamma * pr * rr);
-    const double iSl = rcp(Sl), iSr = rcp(Sr);
+    const double elx = fma(-prl, sl.r, sl.p) * (irl * g1i);
+    const double erx = fma(-prr, sr.r, sr.p) * (irr * g1i);
     // c_x = 0.5 sqrt(gamma/(p rho)) (p rho_x + rho p_x) = 0.5 gamma/S (...)
-    const double clx = 0.5 * gamma * iSl * fma(pl, sl.r, rl * sl.p);
-    const double crx = 0.5 * gamma * iSr * fma(pr, sr.r, rr * sr.p);
+    const double clx = (0.5 * gamma) * iSl * fma(pl, sl.r, rl * sl.p);
+    const double crx = (0.5 * gamma) * iSr * fma(pr, sr.r, rr * sr.p);
     const double du = ul - ur;
-    double cl, cr, icl, icr;
-    if (pr >= pl) {
-        cl = fma(rl * kappa, tmax(0.0, fma(pr - pl, iSr, du)), Sl);
-        icl = rcp(cl);
-        cr = fma(rr * kappa, tmax(0.0, fma(pl - pr, icl, du)), Sr);
-        icr = rcp(cr);
-    } else {
-        cr = fma(rr * kappa, tmax(0.0, fma(pl - pr, iSl, du)), Sr);
-        icr = rcp(cr);
-        cl = fma(rl * kappa, tmax(0.0, fma(pr - pl, icr, du)), Sl);
-        icl = rcp(cl);
-    }
+    // Relaxation speeds (src/basicFuns.cuh:94-104).  The side F with the
+    // lower pressure is relaxed first from the other side's S, then the
+    // other side O from c_F; |p_r - p_l| is the pressure rise seen by F.
+    const bool fL = pr >= pl;
+    const double dp = fabs(pr - pl);
+    const double SF = fL ? Sl : Sr, rF = fL ? rl : rr, iSO = fL ? iSr : iSl;
+    const double SO = fL ? Sr : Sl, rO = fL ? rr : rl;
+    const double cF = fma(rF * kappa, fmax(0.0, fma(dp, iSO, du)), SF);
+    const double icF = rcp(cF);
+    const double cO = fma(rO * kappa, fmax(0.0, fma(-dp, icF, du)), SO);
+    // 1/c_O and 1/(c_l + c_r) from one reciprocal
+    const double csum = cF + cO;
+    const double t = rcp(cO * csum);
+    const double icO = csum * t, ics = cO * t;
+    const double cl = fL ? cF : cO, cr = fL ? cO : cF;
+    const double icl = fL ? icF : icO, icr = fL ? icO : icF;
     const bool supL = ul - cl * irl >= 0.0;
     const bool supR = !supL && ur + cr * irr <= 0.0;
     if (supL || supR) {
@@ -319,7 +342,6 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
         Wt.p = -fma(u, sK.p, cK * cK * irK * sK.un);
         return supL ? 0 : 1;
     }
-    const double ics = rcp(cl + cr);
     const double um = (fma(ul, cl, fma(ur, cr, pl - pr))) * ics;
     const double pm = fma(du * cl, cr, fma(cl, pr, cr * pl)) * ics;
     // k and d use u = 0 (see the exact form)
