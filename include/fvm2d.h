@@ -110,8 +110,10 @@ int fvm2d_sync(fvm2d_solver *s, int *steps_out, double *t_out, double *dt_out);
 /* Wall time of nsteps launches bracketed by HIP events on the solver's
  * stream (after fvm2d_begin).  ms_out: total milliseconds. */
 int fvm2d_time_steps(fvm2d_solver *s, int nsteps, double *ms_out);
-/* Duration of single step-kernel launches: nsteps launches, each bracketed
- * by its own pair of HIP events on the solver's stream.  avg/min in ms. */
+/* Step-kernel launch duration on the solver's stream: avg = elapsed time of
+ * nsteps back-to-back launches between two HIP events / nsteps; min = the
+ * shortest of nsteps further launches each bracketed by its own events.
+ * Both in ms; 2 nsteps steps are advanced. */
 int fvm2d_time_kernel(fvm2d_solver *s, int nsteps, double *avg_ms_out, double *min_ms_out);
 
 /* ---- batched forward (no counterpart in the reference) ----------------- *
