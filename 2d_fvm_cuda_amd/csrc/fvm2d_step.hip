@@ -34,7 +34,7 @@ namespace fvm2d {
 // orientation the reference's rotate_and_flip_vertically produces.
 struct YMarch {
     Q qj, qj1;   // rows j and j+1 (line order)
-    Q sj, fmj;   // slope and south face of row j
+    Q sj;        // slope of row j (its south face is recomputed from it)
     Q Fprev;     // flux through the north face of row j
 };
 
@@ -52,7 +52,6 @@ __device__ __forceinline__ void ymarch_init(YMarch &m, const Q &m2, const Q &m1,
     m.qj = q0;
     m.qj1 = p1;
     m.sj = s_0;
-    m.fmj = face_minus<Exact>(q0, s_0, L);
 }
 
 // Consume row j+2, return the updated row j.
@@ -61,15 +60,14 @@ __device__ __forceinline__ Q ymarch_step(YMarch &m, const Q &qn, const Line &L, 
                                          double dt, double lam) {
     const Q s1 = slope<Exact>(qn, m.qj1, m.qj, L);
     const Q fp1 = face_plus<Exact>(m.qj1, s1, L);
-    const Q fm1 = face_minus<Exact>(m.qj1, s1, L);
+    const Q fmj = face_minus<Exact>(m.qj, m.sj, L);
     IfState W, Wt;
-    grp<Exact>(fp1, m.fmj, s1, m.sj, G, W, Wt);
+    grp<Exact>(fp1, fmj, s1, m.sj, G, W, Wt);
     const Q F = flux<Exact>(W, Wt, dt);               // south face of row j
     const Q out = update<Exact>(m.qj, F, m.Fprev, lam, G);
     m.qj = m.qj1;
     m.qj1 = qn;
     m.sj = s1;
-    m.fmj = fm1;
     m.Fprev = F;
     return out;
 }
