@@ -43,6 +43,7 @@ struct AdjEngine {
     std::map<int, int> stored;       // step -> slot
     // stats
     long long recompute_steps = 0;
+    long long max_cells = 0;  // largest nx * ny over the instances
     int reverse_steps = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0, rev_ms_sum = 0.0;
     hipEvent_t r0 = nullptr, r1 = nullptr;
@@ -76,6 +77,7 @@ struct AdjEngine {
         slots = std::max(1, std::min(nslots <= 0 ? steps : nslots, steps));
         nx.assign(nx_, nx_ + count);
         ny.assign(ny_, ny_ + count);
+        for (int i = 0; i < count; i++) max_cells = std::max(max_cells, (long long)nx[i] * ny[i]);
         HIPCHK(hipEventCreate(&r0));
         HIPCHK(hipEventCreate(&r1));
         had.resize(n);
@@ -92,7 +94,7 @@ struct AdjEngine {
             for (double **b : {&A1[i], &B1[i], &lamA1[i], &lamB1[i], &lamX[i], &lamY[i], &UN[i]})
                 if ((rc = dalloc(b, st))) return rc;
             if ((rc = dalloc(&U0c[i], 4 * (size_t)nx[i] * ny[i]))) return rc;
-            const size_t npairs = ((size_t)nx[i] * ny[i] + 63) / 64;
+            const size_t npairs = (size_t)adj_smax_pairs();
             if ((rc = dalloc(&pmax[i], npairs))) return rc;
             if ((rc = dalloc(&pidx[i], npairs))) return rc;
             HIPCHK(hipMemsetAsync(A1[i], 0, st * sizeof(double), fwd.stream));
@@ -133,8 +135,10 @@ struct AdjEngine {
             const int wa_x = (nx[i] + 55) / 56, wa_y = (ny[i] + 55) / 56;
             for (int j = 0; j < ny[i]; j++)
                 for (int w = 0; w < wf_x; w++) tf.push_back(make_int4(i, 0, w, j));
-            for (int c = 0; c < nx[i]; c++)
-                for (int w = 0; w < wf_y; w++) tf.push_back(make_int4(i, 1, w, c));
+            // Y tasks: adjacent columns consecutive, so the waves of a block
+            // (and of neighbouring blocks) share the cache lines of each row
+            for (int w = 0; w < wf_y; w++)
+                for (int c = 0; c < nx[i]; c++) tf.push_back(make_int4(i, 1, w, c));
         }
         int base = 0;
         for (int i = 0; i < n; i++) {  // R2: X^T on B1 (rows) and Y^T on A1 (columns)
@@ -142,8 +146,8 @@ struct AdjEngine {
             const int wa_x = (nx[i] + 55) / 56, wa_y = (ny[i] + 55) / 56;
             for (int j = 0; j < ny[i]; j++)
                 for (int w = 0; w < wa_x; w++) t2.push_back(make_int4(i, 0, w, j));
-            for (int c = 0; c < nx[i]; c++)
-                for (int w = 0; w < wa_y; w++) t2.push_back(make_int4(i, 1, w, c));
+            for (int w = 0; w < wa_y; w++)
+                for (int c = 0; c < nx[i]; c++) t2.push_back(make_int4(i, 1, w, c));
             had[i].part_hi[0] = base + (int)t2.size();
         }
         base += (int)t2.size();
@@ -160,8 +164,8 @@ struct AdjEngine {
         for (int i = 0; i < n; i++) {  // R3b: Y^T on U (columns)
             had[i].part_lo[2] = base3b + (int)t3b.size();
             const int wa_y = (ny[i] + 55) / 56;
-            for (int c = 0; c < nx[i]; c++)
-                for (int w = 0; w < wa_y; w++) t3b.push_back(make_int4(i, 1, w, c));
+            for (int w = 0; w < wa_y; w++)
+                for (int c = 0; c < nx[i]; c++) t3b.push_back(make_int4(i, 1, w, c));
             had[i].part_hi[2] = base3b + (int)t3b.size();
         }
         base += (int)t3b.size();
@@ -265,8 +269,7 @@ struct AdjEngine {
             had[i].lam_out = lamY[i];
         }
         HIPCHK(hipMemcpyAsync(d_adesc, had.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, fwd.stream));
-        for (int i = 0; i < n; i++) adj_launch_smax(d_adesc, i, (long long)nx[i] * ny[i], fwd.stream);
-        adj_launch_dt(d_adesc, n, fwd.stream);
+        adj_launch_smax(d_adesc, n, fwd.stream);
         adj_launch_fwd(d_adesc, d_tfwd, nfwd, fwd.stream);
         adj_launch_sweep(d_adesc, d_tr2, nr2, 0, 0, fwd.stream);
         adj_launch_sweep(d_adesc, d_tr3a, nr3a, 1, nr2, fwd.stream);

@@ -157,72 +157,73 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
     if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
 }
 
-// s_max over every cell of U_n with the smallest linear index attaining it;
-// the last block turns it into dt.  One thread per cell.
-__global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ descs, int inst) {
-    AdjDesc &d = descs[inst];
-    const long long n = (long long)d.nx * d.ny;
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long mx = 0ull;
-    long long idx = 0x7fffffffffffffffll;
-    if (k < n) {
-        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
-        const long long o = (long long)j * d.pitch + i;
-        const double sp = speed<false>(d.U[o], d.U[d.plane + o], d.U[2 * d.plane + o],
-                                       d.U[3 * d.plane + o], d.gas);
-        mx = bits_of(sp);
-        idx = k;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long m2 = __shfl_xor(mx, o, kWave);
-        const long long i2 = __shfl_xor(idx, o, kWave);
-        if (m2 > mx || (m2 == mx && i2 < idx)) {
-            mx = m2;
-            idx = i2;
-        }
-    }
-    // lane 0 holds the wave's first cell; waves wholly past the mesh own no
-    // pair slot (there are ceil(n / 64) slots)
-    if ((threadIdx.x & (kWave - 1)) == 0 && k < n) {
-        const long long slot = k >> 6;
-        d.pair_max[slot] = mx;
-        d.pair_idx[slot] = idx;
+constexpr int kSmaxBlocks = 64;  // blocks per instance of the s_max kernel
+
+__device__ __forceinline__ void max_pair(unsigned long long &mx, long long &idx, unsigned long long m2,
+                                         long long i2) {
+    if (m2 > mx || (m2 == mx && i2 < idx)) {
+        mx = m2;
+        idx = i2;
     }
 }
 
-// Reduce the per-wave pairs (fixed order), set s_max, arg-max and dt.
-__global__ void __launch_bounds__(256) adj_dt_kernel(AdjDesc *__restrict__ descs, int ninst) {
-    __shared__ unsigned long long smx[256];
-    __shared__ long long sidx[256];
-    for (int inst = 0; inst < ninst; inst++) {
-        AdjDesc &d = descs[inst];
-        const int npairs = (int)(((long long)d.nx * d.ny + kWave - 1) / kWave);
-        unsigned long long mx = 0ull;
-        long long idx = 0x7fffffffffffffffll;
-        for (int p = threadIdx.x; p < npairs; p += blockDim.x) {
-            const unsigned long long m2 = d.pair_max[p];
-            const long long i2 = d.pair_idx[p];
-            if (m2 > mx || (m2 == mx && i2 < idx)) {
-                mx = m2;
-                idx = i2;
-            }
+__device__ __forceinline__ void wave_max_pair(unsigned long long &mx, long long &idx) {
+    for (int o = 32; o > 0; o >>= 1)
+        max_pair(mx, idx, __shfl_xor(mx, o, kWave), __shfl_xor(idx, o, kWave));
+}
+
+// s_max over every cell of U_n with the smallest linear index j * nx + i
+// attaining it (largest bit pattern, then smallest index: the result does
+// not depend on the reduction order), and dt = CFL h / s_max.  blockIdx.y is
+// the instance; each of its kSmaxBlocks blocks strides over rows and leaves
+// one pair, and the last block to arrive reduces the pairs.
+__global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ descs) {
+    __shared__ unsigned long long smx[256 / kWave];
+    __shared__ long long sidx[256 / kWave];
+    __shared__ int last;
+    AdjDesc &d = descs[blockIdx.y];
+    unsigned long long mx = 0ull;
+    long long idx = 0x7fffffffffffffffll;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x) {
+        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+            const long long o = (long long)j * d.pitch + i;
+            const double sp = speed<false>(d.U[o], d.U[d.plane + o], d.U[2 * d.plane + o],
+                                           d.U[3 * d.plane + o], d.gas);
+            max_pair(mx, idx, bits_of(sp), (long long)j * d.nx + i);
         }
-        smx[threadIdx.x] = mx;
-        sidx[threadIdx.x] = idx;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int p = 1; p < (int)blockDim.x; p++)
-                if (smx[p] > mx || (smx[p] == mx && sidx[p] < idx)) {
-                    mx = smx[p];
-                    idx = sidx[p];
-                }
-            const double smax = __longlong_as_double((long long)mx);
-            d.ctl->smax = smax;
-            d.ctl->argmax = idx;
-            d.ctl->dt = d.cfl * d.hmin / smax;
-            d.ctl->status |= (mx > 0x7ff0000000000000ull) ? 1 : 0;
-        }
-        __syncthreads();
+    }
+    wave_max_pair(mx, idx);
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        smx[wv] = mx;
+        sidx[wv] = idx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x / kWave); k++) max_pair(mx, idx, smx[k], sidx[k]);
+        d.pair_max[blockIdx.x] = mx;
+        d.pair_idx[blockIdx.x] = idx;
+        __threadfence();
+        last = atomicAdd(&d.ctl->arrivals, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || wv != 0) return;
+    __threadfence();
+    mx = 0ull;
+    idx = 0x7fffffffffffffffll;
+    for (int k = lane; k < (int)gridDim.x; k += kWave) {
+        // written by other blocks of this launch: read past the caches
+        max_pair(mx, idx, __hip_atomic_load(&d.pair_max[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                 __hip_atomic_load(&d.pair_idx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    wave_max_pair(mx, idx);
+    if (lane == 0) {
+        const double smax = __longlong_as_double((long long)mx);
+        d.ctl->smax = smax;
+        d.ctl->argmax = idx;
+        d.ctl->dt = d.cfl * d.hmin / smax;
+        d.ctl->status |= (mx > 0x7ff0000000000000ull) ? 1 : 0;
+        d.ctl->arrivals = 0u;
     }
 }
 
@@ -322,17 +323,29 @@ __global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ de
 // ---------------------------------------------------------------- R4
 // lam[argmax] += (sum of dt sensitivities) * d dt / d U[argmax], where
 // dt = CFL h / s_max and s_max = speed(U[argmax]).  One block per instance;
-// the sum runs in a fixed order (strided partial sums, then a tree), so the
-// result is reproducible.
-__global__ void __launch_bounds__(256) adj_dt_term_kernel(AdjDesc *__restrict__ descs) {
-    __shared__ double red[256];
+// the sum runs in a fixed order (four strided partial sums per thread, a
+// fixed tree over the block), so the result is reproducible.
+__global__ void __launch_bounds__(1024) adj_dt_term_kernel(AdjDesc *__restrict__ descs) {
+    __shared__ double red[1024];
     AdjDesc &d = descs[blockIdx.x];
-    double acc = 0.0;
-    for (int r = 0; r < 3; r++)
-        for (int p = d.part_lo[r] + (int)threadIdx.x; p < d.part_hi[r]; p += blockDim.x) acc += d.dtpart[p];
-    red[threadIdx.x] = acc;
+    const int T = blockDim.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int r = 0; r < 3; r++) {
+        const int lo = d.part_lo[r], hi = d.part_hi[r];
+        int p = lo + (int)threadIdx.x;
+        for (; p + 3 * T < hi; p += 4 * T) {
+            a0 += d.dtpart[p];
+            a1 += d.dtpart[p + T];
+            a2 += d.dtpart[p + 2 * T];
+            a3 += d.dtpart[p + 3 * T];
+        }
+        if (p < hi) a0 += d.dtpart[p];
+        if (p + T < hi) a1 += d.dtpart[p + T];
+        if (p + 2 * T < hi) a2 += d.dtpart[p + 2 * T];
+    }
+    red[threadIdx.x] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    for (int w = T / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -424,12 +437,10 @@ static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n +
 void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s) {
     hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks, ntasks);
 }
-void adj_launch_smax(AdjDesc *descs, int inst, long long ncells, hipStream_t s) {
-    hipLaunchKernelGGL(adj_smax_kernel, dim3(blocks_for(ncells, 256)), dim3(256), 0, s, descs, inst);
+void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
+    hipLaunchKernelGGL(adj_smax_kernel, dim3(kSmaxBlocks, ninst), dim3(256), 0, s, descs);
 }
-void adj_launch_dt(AdjDesc *descs, int ninst, hipStream_t s) {
-    hipLaunchKernelGGL(adj_dt_kernel, dim3(1), dim3(256), 0, s, descs, ninst);
-}
+int adj_smax_pairs() { return kSmaxBlocks; }
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
                       hipStream_t s) {
     hipLaunchKernelGGL(adj_sweep_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks,
@@ -437,7 +448,7 @@ void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, i
 }
 void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s) {
     (void)nparts;
-    hipLaunchKernelGGL(adj_dt_term_kernel, dim3(ninst), dim3(256), 0, s, descs);
+    hipLaunchKernelGGL(adj_dt_term_kernel, dim3(ninst), dim3(1024), 0, s, descs);
 }
 void adj_launch_cons_to_prim(const double *cons, double *prim, int nx, int ny, int pitch, long long plane,
                              double g1, hipStream_t s) {

@@ -124,12 +124,17 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
     const double rlx = sl.r, plx = sl.p, ulx = sl.un, vlx = sl.ut;
     const double rrx = sr.r, prx = sr.p, urx = sr.un, vrx = sr.ut;
     // ---- forward (same statements as grp<false>)
-    const double irl = 1.0 / rl, irr = 1.0 / rr;
+    // S = sqrt(gamma p rho), 1/S and 1/rho = gamma p / S^2 from one
+    // reciprocal square root per side (the reverse below uses only the
+    // mathematical relations between these quantities)
+    const double gpl = gamma * pl, gpr = gamma * pr;
+    const double Xl = gpl * rl, Xr = gpr * rr;
+    const double iSl = rsq(Xl), iSr = rsq(Xr);
+    const double Sl = Xl * iSl, Sr = Xr * iSr;
+    const double irl = gpl * iSl * iSl, irr = gpr * iSr * iSr;
     const double el = pl * irl * g1i, er = pr * irr * g1i;
     const double zl = plx - pl * irl * rlx, zr = prx - pr * irr * rrx;
     const double elx = zl * irl * g1i, erx = zr * irr * g1i;
-    const double Sl = sqrt(gamma * pl * rl), Sr = sqrt(gamma * pr * rr);
-    const double iSl = 1.0 / Sl, iSr = 1.0 / Sr;
     const double sxl = pl * rlx + rl * plx, sxr = pr * rrx + rr * prx;
     const double clx = 0.5 * gamma * iSl * sxl, crx = 0.5 * gamma * iSr * sxr;
     const double du = ul - ur;
@@ -139,20 +144,20 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
         xl = (pr - pl) * iSr + du;
         ml = tmax(0.0, xl);
         cl = Sl + rl * kappa * ml;
-        icl = 1.0 / cl;
+        icl = rcp(cl);
         xr = (pl - pr) * icl + du;
         mr = tmax(0.0, xr);
         cr = Sr + rr * kappa * mr;
-        icr = 1.0 / cr;
+        icr = rcp(cr);
     } else {
         xr = (pl - pr) * iSl + du;
         mr = tmax(0.0, xr);
         cr = Sr + rr * kappa * mr;
-        icr = 1.0 / cr;
+        icr = rcp(cr);
         xl = (pr - pl) * icr + du;
         ml = tmax(0.0, xl);
         cl = Sl + rl * kappa * ml;
-        icl = 1.0 / cl;
+        icl = rcp(cl);
     }
     // ---- adjoints of intermediates
     double b_rl = 0, b_pl = 0, b_ul = 0, b_vl = 0, b_rr = 0, b_pr = 0, b_ur = 0, b_vr = 0;
@@ -219,7 +224,7 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
         b_urx -= cr * cr * irr * Wtb.p;
     } else {  // star region
         // ---- forward of the common star part
-        const double ics = 1.0 / (cl + cr);
+        const double ics = rcp(cl + cr);
         const double Nu = ul * cl + ur * cr + pl - pr;
         const double um = Nu * ics;
         const double Np = du * cl * cr + cl * pr + cr * pl;
@@ -238,7 +243,7 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
         if (um <= 0.0) {
             // ---- forward of the right sub-case
             const double den = ur - um + cr * irr;
-            const double iden = 1.0 / den;
+            const double iden = rcp(den);
             const double rho = cr * iden;
             const double irho = den * icr;
             const double icr2 = icr * icr;
@@ -364,7 +369,7 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
         } else {
             // ---- forward of the left sub-case
             const double den = um - ul + cl * irl;
-            const double iden = 1.0 / den;
+            const double iden = rcp(den);
             const double rho = cl * iden;
             const double irho = den * icl;
             const double icl2 = icl * icl;
@@ -645,7 +650,7 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
 // adjoint of q, and returns dq'/dlambda . qb' through lamb.
 FVM_HD Q prim_adj(const Q &qo, const Q &qob, const Gas &G) {
     // q' = (c0, g1 (c2 - (c1^2 + c3^2)/(2 c0)), c1/c0, c3/c0) at q' = qo
-    const double ir = 1.0 / qo.r;
+    const double ir = rcp(qo.r);
     const double un = qo.un, ut = qo.ut;
     Q Ub;
     Ub.r = qob.r + qob.p * G.g1 * 0.5 * (un * un + ut * ut) - (qob.un * un + qob.ut * ut) * ir;

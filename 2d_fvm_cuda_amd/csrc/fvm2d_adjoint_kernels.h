@@ -14,7 +14,7 @@ struct AdjCtl {
     double dtbar;      // total sensitivity of the loss to dt_n
     long long argmax;  // linear index (j * nx + i) of the first cell attaining s_max
     int status;        // bit 0: non-finite wave speed
-    int pad;
+    unsigned int arrivals;  // blocks of the s_max kernel done (0 between steps)
 };
 
 // Task of the line kernels: x = instance, y = 0 for an X line (a row) or 1
@@ -38,8 +38,10 @@ struct AdjDesc {
 };
 
 void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s);
-void adj_launch_smax(AdjDesc *descs, int inst, long long ncells, hipStream_t s);
-void adj_launch_dt(AdjDesc *descs, int ninst, hipStream_t s);
+// s_max, its arg-max and dt of every instance (one launch); the pair arrays
+// hold adj_smax_pairs() entries per instance.
+void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s);
+int adj_smax_pairs();
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
                       hipStream_t s);
 void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s);

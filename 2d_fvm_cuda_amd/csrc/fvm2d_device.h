@@ -18,6 +18,18 @@ enum Strategy : int {
     kDoubleMach = 4,
 };
 
+// Arrival slots of the step kernel's end-of-task reduction.  The tasks of
+// an instance are spread over kCtlSlots slots (task index mod kCtlSlots),
+// each on its own 128-byte line; the last task of a slot forwards the slot's
+// maximum to the instance's Ctl.  Without the slots every wave of a launch
+// (about 2000 at 2000 x 2000) would meet on one address at the end.
+constexpr int kCtlSlots = 32;
+struct alignas(128) CtlSlot {
+    unsigned long long smax;  // bit-pattern maximum of the slot's tasks
+    unsigned int count;       // tasks of the slot finished in this launch
+    unsigned int pad[29];
+};
+
 // One mesh ("instance").  State lives in two ping-pong buffers of four
 // planes each (rho, p, u, v), row-major with a padded row pitch; the buffer
 // holding the current state is buf[steps & 1].
@@ -36,6 +48,7 @@ struct InstDesc {
     double t_end, cfl, hmin;  // hmin = min(width/nx, height/ny)
     Line lx, ly;
     Gas gas;
+    CtlSlot *slots;    // kCtlSlots arrival slots, zero between launches
 };
 
 // Per-instance time-loop control, resident on the device.

@@ -70,6 +70,7 @@ struct Engine {
     Ctl *d_ctl = nullptr;
     Ctl *h_ctl = nullptr;            // pinned read-back
     Sched *d_sched = nullptr;
+    CtlSlot *d_slots = nullptr;      // n * kCtlSlots end-of-task arrival slots
     double *d_t0 = nullptr, *h_t0 = nullptr;  // per-instance start times
     int2 *d_tasks = nullptr;
     int ntasks = 0;
@@ -93,6 +94,8 @@ struct Engine {
         if (d_desc) hipFree(d_desc);
         if (d_ctl) hipFree(d_ctl);
         if (d_sched) hipFree(d_sched);
+        if (d_slots) hipFree(d_slots);
+        d_slots = nullptr;
         if (d_tasks) hipFree(d_tasks);
         if (h_ctl) hipHostFree(h_ctl);
         if (d_t0) hipFree(d_t0);
@@ -174,6 +177,9 @@ struct Engine {
         HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
         HIPCHK(hipMalloc(&d_sched, sizeof(Sched)));
         HIPCHK(hipMemsetAsync(d_sched, 0, sizeof(Sched), stream));
+        HIPCHK(hipMalloc(&d_slots, (size_t)n * kCtlSlots * sizeof(CtlSlot)));
+        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
+        for (int i = 0; i < n; i++) hd[i].slots = d_slots + (size_t)i * kCtlSlots;
         HIPCHK(hipHostMalloc(&h_ctl, n * sizeof(Ctl), hipHostMallocDefault));
         HIPCHK(hipMalloc(&d_t0, n * sizeof(double)));
         HIPCHK(hipHostMalloc(&h_t0, n * sizeof(double), hipHostMallocDefault));
@@ -299,8 +305,8 @@ struct Engine {
     // t0: per-instance start times (host array of n values), or null for 0.
     int begin(const double *t0 = nullptr) {
         HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
-        for (int i = 0; i < n; i++)
-            launch_speed(exact, d_desc, d_ctl, i, 0, (long long)hd[i].nx * hd[i].ny, stream);
+        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
+        for (int i = 0; i < n; i++) launch_speed(exact, d_desc, d_ctl, i, 0, hd[i].ny, stream);
         if (t0) {
             for (int i = 0; i < n; i++) h_t0[i] = t0[i];
             HIPCHK(hipMemcpyAsync(d_t0, h_t0, n * sizeof(double), hipMemcpyHostToDevice, stream));

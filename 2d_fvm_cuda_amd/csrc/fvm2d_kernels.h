@@ -10,7 +10,7 @@ namespace fvm2d {
 void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream);
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
-                  long long ncells, hipStream_t stream);
+                  long long nrows, hipStream_t stream);
 void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double *t0s,
                       hipStream_t stream);
 int step_kernel_occupancy(bool exact, int wpb);

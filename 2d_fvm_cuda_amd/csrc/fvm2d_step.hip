@@ -256,13 +256,29 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
 
     mx = wave_max_u64(mx);
     if (lane == 0) {
-        atomicMax(&ctl->smax_next, mx);
+        // two-level arrival: task -> its slot -> the instance
+        const int nt = d.nw * d.ns;
+        const int slot = local % kCtlSlots;
+        const int nslots = nt < kCtlSlots ? nt : kCtlSlots;
+        const int per = nt / kCtlSlots + (slot < nt % kCtlSlots ? 1 : 0);
+        CtlSlot *sl = d.slots + slot;
+        atomicMax(&sl->smax, mx);
         __threadfence();
-        const unsigned int prev = atomicAdd(&ctl->waves_done, 1u);
-        if (prev == (unsigned int)(d.nw * d.ns - 1)) {
+        const unsigned int prev = atomicAdd(&sl->count, 1u);
+        if (prev == (unsigned int)(per - 1)) {
+            // last task of the slot: take the slot's maximum and clear the
+            // slot for the next launch
             __threadfence();
-            const unsigned long long sb = atomicOr(&ctl->smax_next, 0ull);
-            advance_ctl(d, ctl, t_new, steps + 1, sb);
+            const unsigned long long sm = atomicExch(&sl->smax, 0ull);
+            atomicExch(&sl->count, 0u);
+            atomicMax(&ctl->smax_next, sm);
+            __threadfence();
+            const unsigned int prev2 = atomicAdd(&ctl->waves_done, 1u);
+            if (prev2 == (unsigned int)(nslots - 1)) {
+                __threadfence();
+                const unsigned long long sb = atomicOr(&ctl->smax_next, 0ull);
+                advance_ctl(d, ctl, t_new, steps + 1, sb);
+            }
         }
     }
 }
@@ -306,26 +322,33 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
 
 // ---------------------------------------------------------------- set-up
 // s_max over every cell of the current state (ghosts included), as
-// maxPropagtingSpeed + thrust::reduce do at src/wrappers.cuh:198-201.
+// maxPropagtingSpeed + thrust::reduce do at src/wrappers.cuh:198-201.  Blocks
+// stride over rows, threads over columns; one atomic per block.
 template <bool Exact>
 __global__ void __launch_bounds__(256) fvm_speed_kernel(const InstDesc *__restrict__ descs,
                                                         Ctl *__restrict__ ctls, int inst,
                                                         int use_buf1) {
+    __shared__ unsigned long long red[256 / kWave];
     const InstDesc d = descs[inst];
     const double *src = use_buf1 ? d.buf1 : d.buf0;
-    const long long n = (long long)d.nx * d.ny;
     unsigned long long mx = 0ull;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
-        const long long o = (long long)j * d.pitch + i;
-        const double sp = speed<Exact>(src[o], src[d.plane + o], src[2 * d.plane + o],
-                                       src[3 * d.plane + o], d.gas);
-        const unsigned long long b = dbits(sp);
-        mx = b > mx ? b : mx;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x) {
+        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+            const long long o = (long long)j * d.pitch + i;
+            const double sp = speed<Exact>(src[o], src[d.plane + o], src[2 * d.plane + o],
+                                           src[3 * d.plane + o], d.gas);
+            const unsigned long long b = dbits(sp);
+            mx = b > mx ? b : mx;
+        }
     }
     mx = wave_max_u64(mx);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&ctls[inst].smax_next, mx);
+    const int wv = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) red[wv] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x / kWave); k++) mx = red[k] > mx ? red[k] : mx;
+        atomicMax(&ctls[inst].smax_next, mx);
+    }
 }
 
 __global__ void fvm_ctl_start_kernel(const InstDesc *__restrict__ descs, Ctl *__restrict__ ctls,
@@ -353,9 +376,9 @@ void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks
 }
 
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
-                  long long ncells, hipStream_t stream) {
-    long long blocks = (ncells + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+                  long long nrows, hipStream_t stream) {
+    long long blocks = nrows;
+    if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
     if (exact)
         hipLaunchKernelGGL(fvm_speed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, stream,

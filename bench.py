@@ -64,6 +64,8 @@ def main():
                     help="strip_rows,waves_per_block,persistent: fixed geometry instead of autotuning")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
+    ap.add_argument("--leg", choices=["batched", "adjoint", "batched_adjoint", "peak_8192"],
+                    help="run only this extra leg and print its record")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 256 MiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
@@ -84,6 +86,13 @@ def main():
         ms = fvm.hbm_copy_probe(n_doubles, 5)
         print(json.dumps({"copy_probe_bytes_read": COPY_PROBE_BYTES, "copy_probe_bytes_written": COPY_PROBE_BYTES,
                           "avg_ms": ms, "GB_s": 2 * COPY_PROBE_BYTES / (ms / 1000.0) / 1e9}))
+        return
+
+    if args.leg:
+        extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
+        fn = {"batched": extras.batched, "adjoint": extras.adjoint,
+              "batched_adjoint": extras.batched_adjoint, "peak_8192": extras.peak}[args.leg]
+        print(json.dumps({args.leg: fn(fvm)}))
         return
 
     N = args.n
