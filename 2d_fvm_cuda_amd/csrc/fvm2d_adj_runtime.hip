@@ -48,6 +48,7 @@ struct AdjEngine {
     double fwd_ms = 0.0, bwd_ms = 0.0, rev_ms_sum = 0.0;
     hipEvent_t r0 = nullptr, r1 = nullptr;
     bool have_forward = false;
+    bool launch_chosen = false;  // the forward engine's launch geometry is fixed
 
     ~AdjEngine() {
         for (void *p : allocs) hipFree(p);
@@ -333,6 +334,14 @@ struct AdjEngine {
                                     gamma[i] - 1.0, fwd.stream);
         }
         HIPCHK(hipGetLastError());
+        if (!launch_chosen) {
+            // first forward call: time the candidate launch geometries of the
+            // step kernel on this initial state (the autotuner restores it)
+            int rc = copy_state(0, -1);
+            if (!rc) rc = fwd.autotune(2, nullptr);
+            if (rc) return rc;
+            launch_chosen = true;
+        }
         slot_t.assign(slots, std::vector<double>(n, 0.0));
         slot_step.assign(slots, -1);
         stored.clear();
@@ -512,6 +521,7 @@ int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *st) {
 
 int fvm2d_adj_set_launch(fvm2d_adj *a, int strip_rows, int waves_per_block, int persistent) {
     if (!a) return fail(FVM2D_ERR_ARG, "adjoint is null");
+    a->eng.launch_chosen = true;
     return a->eng.fwd.set_launch(strip_rows, waves_per_block, persistent);
 }
 

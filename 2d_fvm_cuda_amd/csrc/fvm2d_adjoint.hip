@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
     if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
 }
 
-constexpr int kSmaxBlocks = 64;  // blocks per instance of the s_max kernel
+constexpr int kSmaxBlocks = 256;  // blocks per instance of the s_max kernel
 
 __device__ __forceinline__ void max_pair(unsigned long long &mx, long long &idx, unsigned long long m2,
                                          long long i2) {

@@ -9,6 +9,14 @@
 
 #include "fvm2d_cell.h"
 
+// The reverse statements are sums of products: on the device they may
+// contract to fused multiply-adds (the file is not part of the bitwise
+// EXACT path).  The host build of the reference gradient keeps IEEE
+// operations; the two agree to rounding.
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma clang fp contract(fast)
+#endif
+
 namespace fvm2d {
 
 // ---------------------------------------------------------------- limiter

@@ -171,6 +171,8 @@ int fvm2d_adj_backward(fvm2d_adj *a, const double *const *d_gN_cons, double *con
 int fvm2d_adj_forward_host(fvm2d_adj *a, const double *const *U0_cons, double *const *UN_cons);
 int fvm2d_adj_backward_host(fvm2d_adj *a, const double *const *gN_cons, double *const *g0_cons);
 int fvm2d_adj_get_stats(const fvm2d_adj *a, fvm2d_adj_stats *out);
+/* Launch geometry of the forward step kernel (as fvm2d_set_launch).  Without
+ * it the first forward call autotunes the geometry on its initial state. */
 int fvm2d_adj_set_launch(fvm2d_adj *a, int strip_rows, int waves_per_block, int persistent);
 int fvm2d_adj_destroy(fvm2d_adj *a);
 

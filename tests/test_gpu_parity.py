@@ -5,6 +5,10 @@ bit for bit; FAST mode (shared reciprocals, FMA) must stay within relative
 L1 <= 1e-10 on (rho, rho u, rho v, E) -- the tolerance the plain solve is
 held to.  All calls go through the C ABI of libfvm2d.so.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -77,6 +81,27 @@ def test_config1_256(gpu, orc, problems):
     l1, linf = rel_errors(fa, ref)
     print(f"256^2 fast: steps {steps_fa} relL1 {l1} relLinf {linf}")
     assert steps_fa == steps_ref
+    assert np.all(l1 <= TOL_L1)
+
+
+def test_config2_2000(gpu, problems):
+    """2000^2 four-quadrant to t_end = 0.3, the headline workload, against
+    the oracle's committed record (tests/golden/make_config2.py): EXACT mode
+    bit for bit (step count, SHA-256 of the final (rho, p, u, v) planes and
+    every 16th cell), FAST mode within relative L1 1e-10 of it."""
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    golden = json.load(open(os.path.join(gdir, "config2_2000.json")))
+    sub16 = np.load(os.path.join(gdir, "config2_2000_sub16.npy"))
+    U0 = problems.initialize_4shocks(2000)
+    ex, steps_ex, _ = gpu.solve(U0, mode=gpu.MODE_EXACT, t_end=0.3)
+    assert steps_ex == golden["steps"] == 3716
+    P4 = np.ascontiguousarray(ex[..., :4])
+    assert np.array_equal(P4[::16, ::16], sub16)
+    assert hashlib.sha256(P4.tobytes()).hexdigest() == golden["sha256_prim4"]
+    fa, steps_fa, _ = gpu.solve(U0, mode=gpu.MODE_FAST, t_end=0.3)
+    l1, linf = rel_errors(fa, ex)
+    print(f"2000^2 fast vs exact: steps {steps_fa} relL1 {l1} relLinf {linf}")
+    assert steps_fa == golden["steps"]
     assert np.all(l1 <= TOL_L1)
 
 
