@@ -67,7 +67,7 @@ EXPORTS = [
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
-    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe",
+    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe",
     "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
     "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
@@ -119,6 +119,7 @@ def lib():
         "fvm2d_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "fvm2d_batch_destroy": (ctypes.c_int, [_vp]),
         "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
+        "fvm2d_valu_probe": (ctypes.c_int, [ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -139,6 +140,14 @@ def check(rc: int, what: str = "") -> None:
     if rc != FVM2D_OK:
         msg = lib().fvm2d_last_error().decode(errors="replace")
         raise FVM2DError(f"{what} failed ({rc}): {msg}")
+
+
+def valu_probe(kind: int) -> float:
+    """Wave-level instructions per SIMD per second of one VALU kind (0 fma_f64,
+    1 mul_f64, 2 add_f64, 3 rcp_f64, 4 32-bit integer ops)."""
+    r = ctypes.c_double(0.0)
+    check(lib().fvm2d_valu_probe(int(kind), ctypes.byref(r)), "fvm2d_valu_probe")
+    return r.value
 
 
 def device_available() -> bool:

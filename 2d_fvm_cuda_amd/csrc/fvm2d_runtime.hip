@@ -363,3 +363,71 @@ int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out) {
 }
 
 }  // extern "C"
+
+// VALU throughput probe: 16 independent chains per lane of one instruction
+// kind, 2048 blocks of 256 threads (8 waves per CU), `iters` iterations.
+template <int OP>
+__global__ void __launch_bounds__(256) valu_probe_kernel(double *out, double b, double c, int iters) {
+    double a[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) a[k] = threadIdx.x * 1e-3 + k;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (OP == 0) a[k] = fma(a[k], b, c);
+            if (OP == 1) a[k] = a[k] * b;
+            if (OP == 2) a[k] = a[k] + c;
+            if (OP == 3) a[k] = __builtin_amdgcn_rcp(a[k]);
+            if (OP == 4) a[k] = __hiloint2double(__double2hiint(a[k]) ^ 0x10, __double2loint(a[k]) + 3);
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += a[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+extern "C" {
+
+int fvm2d_valu_probe(int kind, double *per_simd_per_s) {
+    if (kind < 0 || kind > 4) return fail(FVM2D_ERR_ARG, "probe kind must be 0..4");
+    if (int rc0 = device_ok()) return rc0;
+    const int grid = 2048, block = 256, iters = 4096;
+    double *out = nullptr;
+    HIPCHK(hipMalloc((void **)&out, (size_t)grid * block * sizeof(double)));
+    hipEvent_t e0, e1;
+    int rc = FVM2D_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        hipFree(out);
+        return fail(FVM2D_ERR_HIP, "probe setup failed");
+    }
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && !rc; rep++) {
+        hipEventRecord(e0, nullptr);
+        switch (kind) {
+            case 0: valu_probe_kernel<0><<<grid, block>>>(out, 1.0000001, 1e-9, iters); break;
+            case 1: valu_probe_kernel<1><<<grid, block>>>(out, 1.0000001, 1e-9, iters); break;
+            case 2: valu_probe_kernel<2><<<grid, block>>>(out, 1.0000001, 1e-9, iters); break;
+            case 3: valu_probe_kernel<3><<<grid, block>>>(out, 1.0000001, 1e-9, iters); break;
+            default: valu_probe_kernel<4><<<grid, block>>>(out, 1.0000001, 1e-9, iters); break;
+        }
+        hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FVM2D_ERR_HIP, "probe failed");
+        hipEventElapsedTime(&ms, e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(out);
+    if (!rc && per_simd_per_s) {
+        // wave-level instructions of the probed kind (kind 4: two 32-bit ops per chain step)
+        const double waves = (double)grid * block / 64.0;
+        const double ops = waves * iters * 16.0 * (kind == 4 ? 2.0 : 1.0);
+        *per_simd_per_s = ops / (ms * 1e-3) / (4.0 * cus);
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -66,6 +66,8 @@ def main():
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
     ap.add_argument("--leg", choices=["batched", "adjoint", "batched_adjoint", "peak_8192"],
                     help="run only this extra leg and print its record")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU throughput probe (instructions per SIMD per second)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 256 MiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
@@ -80,6 +82,11 @@ def main():
     if not fvm.device_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.valu_probe:
+        names = ["v_fma_f64", "v_mul_f64", "v_add_f64", "v_rcp_f64", "int32"]
+        print(json.dumps({"valu_per_simd_per_s": {nm: fvm.valu_probe(k) for k, nm in enumerate(names)}}))
+        return
 
     if args.copy_probe:
         n_doubles = COPY_PROBE_BYTES // 8

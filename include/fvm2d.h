@@ -142,6 +142,13 @@ int fvm2d_batch_destroy(fvm2d_batch *b);
  * mean duration of one copy. */
 int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out);
 
+/* Diagnostic with no counterpart in the reference: sustained wave-level
+ * instruction rate per SIMD of one VALU kind with 16 independent chains per
+ * lane on every SIMD (kind 0 v_fma_f64, 1 v_mul_f64, 2 v_add_f64,
+ * 3 v_rcp_f64, 4 32-bit integer ops).  Bounds what an FP64-issue-bound
+ * kernel can reach on the card at its sustained clock. */
+int fvm2d_valu_probe(int kind, double *per_simd_per_s);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- *
  * Hand-written reverse mode of the full split step (limiter and GRP
  * differentiated along the branch taken, dt differentiated through its
