@@ -404,9 +404,47 @@ class RGRP_2D_CUDA:
     def solve(self):
         if self._U0 is None:
             raise FVM2DError("no initial data")
+        if self._record:
+            return self._solve_recording()
         U, steps, ms = solve(self._U0, width=self._width, height=self._height,
                              t_end=self._endTime, cfl=self._CFL, alpha=self._alpha,
                              gamma=self._gamma, strategy=int(self._strategy), mode=self._mode)
+        self._U0 = U
+        self._U = U
+        self._steps += steps
+        self._ms = ms
+        return U
+
+    def _solve_recording(self):
+        """Record mode (src/wrappers.cuh:253-261): a frame of the density
+        after the first step and after every gapPerFrame further steps,
+        named by the index of the step (0, gap, 2 gap, ...).  The loop runs
+        on the device in chunks of gapPerFrame steps with a download per
+        frame; the frame holds the state after that step's ghost fill (the
+        reference copies it out just before the fill)."""
+        import time
+        from .io import write_frame_pgm
+        gap = max(1, self._gap)
+        s = Solver(self._Nx, self._Ny, width=self._width, height=self._height,
+                   t_end=self._endTime, cfl=self._CFL, alpha=self._alpha, gamma=self._gamma,
+                   strategy=int(self._strategy), mode=self._mode)
+        try:
+            s.upload(self._U0)
+            t0 = time.perf_counter()
+            s.begin()
+            done_steps, due = 0, 1
+            while True:
+                s.advance(due - done_steps)
+                steps, _, _ = s.sync()
+                if steps < due:
+                    break
+                done_steps = steps
+                write_frame_pgm(self._recordDir, s.download()[..., 0], due - 1)
+                due += gap
+            ms = (time.perf_counter() - t0) * 1e3
+            U = s.download()
+        finally:
+            s.close()
         self._U0 = U
         self._U = U
         self._steps += steps

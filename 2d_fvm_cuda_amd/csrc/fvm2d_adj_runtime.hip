@@ -246,6 +246,7 @@ struct AdjEngine {
             rc = fwd.upload_desc();
             if (rc) return rc;
         }
+        fwd.begun = false;  // the state is back in buf0
         recompute_steps += m;
         if (t_out) {
             t_out->resize(n);

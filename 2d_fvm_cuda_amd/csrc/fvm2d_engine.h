@@ -267,10 +267,27 @@ struct Engine {
         return FVM2D_OK;
     }
 
+    // Buffer holding instance i's current state: buf0 between time loops;
+    // inside one (after begin, before finish_run) the step kernel has
+    // written buf[steps & 1], steps counted from begin.
+    int current_buffer(int i, const double **src) {
+        const InstDesc &d = hd[i];
+        *src = d.buf0;
+        if (begun) {
+            int rc = read_ctl();
+            if (rc) return rc;
+            if (h_ctl[i].steps & 1) *src = d.buf1;
+        }
+        return FVM2D_OK;
+    }
+
     int download(int i, double *U5) {
         if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
         const InstDesc &d = hd[i];
-        HIPCHK(hipMemcpyAsync(stage, d.buf0, 4 * d.plane * sizeof(double), hipMemcpyDeviceToHost, stream));
+        const double *src = nullptr;
+        int rc = current_buffer(i, &src);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(stage, src, 4 * d.plane * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         for (int j = 0; j < d.ny; j++) {
             double *o = U5 + (long long)j * d.nx * 5;
@@ -285,8 +302,13 @@ struct Engine {
     int planes_device(int i, double *d_planes, bool in) {
         if (i < 0 || i >= n || !d_planes) return fail(FVM2D_ERR_ARG, "bad instance or null pointer");
         const InstDesc &d = hd[i];
+        const double *cur = d.buf0;
+        if (!in) {
+            int rc = current_buffer(i, &cur);
+            if (rc) return rc;
+        }
         for (int k = 0; k < 4; k++) {
-            double *a = d.buf0 + k * d.plane;
+            double *a = (in ? d.buf0 : const_cast<double *>(cur)) + k * d.plane;
             double *b = d_planes + (long long)k * d.nx * d.ny;
             if (in)
                 HIPCHK(hipMemcpy2DAsync(a, d.pitch * sizeof(double), b, d.nx * sizeof(double),

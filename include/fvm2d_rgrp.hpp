@@ -8,11 +8,18 @@
 //   ghostCellStrategy  <- src/wrappers.cuh:17-24 (same enumerators, same order)
 //   RGRP_2D_CUDA       <- src/wrappers.cuh:26-65 (same methods and defaults)
 //
-// Visualisation (setRecordModel's PNG frames, src/wrappers.cuh:323-345) needs
-// OpenCV, which this build does not use: recording only stores the settings.
+// Record mode (setRecordModel, src/wrappers.cuh:147-152,253-261,323-345)
+// writes a frame of the density after the first step and every gapPerFrame
+// steps, named by the step index, as an 8-bit binary PGM <index>.pgm (the
+// min-max normalised image the reference colours with viridis and saves as
+// PNG through OpenCV, which this build does not use).
 #pragma once
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstddef>
+#include <filesystem>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -103,8 +110,12 @@ class RGRP_2D_CUDA {
         p.mode = mode_;
         int steps = 0;
         double ms = 0.0;
-        const int rc = fvm2d_solve(reinterpret_cast<double *>(U0_.data()), Nx_, Ny_, &p, &steps, &ms);
-        if (rc != FVM2D_OK) throw std::runtime_error(std::string("fvm2d_solve: ") + fvm2d_last_error());
+        if (record_) {
+            solve_recording(p, &steps, &ms);
+        } else {
+            const int rc = fvm2d_solve(reinterpret_cast<double *>(U0_.data()), Nx_, Ny_, &p, &steps, &ms);
+            if (rc != FVM2D_OK) throw std::runtime_error(std::string("fvm2d_solve: ") + fvm2d_last_error());
+        }
         steps_ += steps;
         ms_ = ms;
         U_ = vectorized();
@@ -128,6 +139,68 @@ class RGRP_2D_CUDA {
     }
 
   protected:
+    static void check(int rc, const char *what) {
+        if (rc != FVM2D_OK) throw std::runtime_error(std::string(what) + ": " + fvm2d_last_error());
+    }
+
+    // The time loop on the device in chunks of gap_ steps; after the first
+    // step and every gap_ further steps the state is copied out and its
+    // density written as <recordDir_>/<step index>.pgm.  The frame holds the
+    // state after that step's ghost fill (the reference copies it out just
+    // before the fill).
+    void solve_recording(const fvm2d_params &p, int *steps_out, double *ms_out) {
+        fvm2d_solver *sv = nullptr;
+        check(fvm2d_create(Nx_, Ny_, &p, &sv), "fvm2d_create");
+        std::vector<vec5d> frame((size_t)Nx_ * Ny_);
+        int steps = 0;
+        try {
+            check(fvm2d_upload(sv, reinterpret_cast<const double *>(U0_.data())), "fvm2d_upload");
+            std::filesystem::create_directories(recordDir_);
+            const auto t0 = std::chrono::steady_clock::now();
+            check(fvm2d_begin(sv), "fvm2d_begin");
+            const int gap = std::max(1, gap_);
+            int due = 1, done = 0;
+            for (;;) {
+                check(fvm2d_advance(sv, due - done), "fvm2d_advance");
+                double t = 0.0, dt = 0.0;
+                check(fvm2d_sync(sv, &steps, &t, &dt), "fvm2d_sync");
+                if (steps < due) break;
+                done = steps;
+                check(fvm2d_download(sv, reinterpret_cast<double *>(frame.data())), "fvm2d_download");
+                write_frame(frame, due - 1);
+                due += gap;
+            }
+            *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            check(fvm2d_download(sv, reinterpret_cast<double *>(U0_.data())), "fvm2d_download");
+        } catch (...) {
+            fvm2d_destroy(sv);
+            throw;
+        }
+        fvm2d_destroy(sv);
+        *steps_out = steps;
+    }
+
+    // Density min-max normalised to 0..255 and rounded (the
+    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340).
+    void write_frame(const std::vector<vec5d> &f, int index) const {
+        double lo = f[0](0), hi = f[0](0);
+        for (const auto &e : f) {
+            lo = std::min(lo, e(0));
+            hi = std::max(hi, e(0));
+        }
+        const double scale = hi > lo ? 255.0 / (hi - lo) : 0.0;
+        std::ofstream out(recordDir_ + "/" + std::to_string(index) + ".pgm", std::ios::binary);
+        out << "P5\n" << Nx_ << " " << Ny_ << "\n255\n";
+        std::vector<unsigned char> row((size_t)Nx_);
+        for (int j = 0; j < Ny_; ++j) {
+            for (int i = 0; i < Nx_; ++i) {
+                const double v = std::nearbyint(f[(size_t)j * Nx_ + i](0) * scale - lo * scale);
+                row[i] = (unsigned char)std::min(255.0, std::max(0.0, v));
+            }
+            out.write(reinterpret_cast<const char *>(row.data()), (std::streamsize)row.size());
+        }
+    }
+
     std::vector<std::vector<vec5d>> vectorized() const {
         std::vector<std::vector<vec5d>> r(Ny_);
         for (int j = 0; j < Ny_; ++j)

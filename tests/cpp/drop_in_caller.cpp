@@ -1,5 +1,6 @@
 // test2D-style caller (test2D.cu:46-80, 135-152) of the drop-in header.
-// Usage: drop_in_caller N t_end  -> prints "steps ms rho_sum"
+// Usage: drop_in_caller N t_end [mode [record_dir gap]]
+//   -> prints "steps ms rho_sum"
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -31,6 +32,7 @@ int main(int argc, char **argv) {
     solver.setAlpha(1.9);
     solver.setGamma(1.4);
     if (argc > 3) solver.setArithmeticMode(std::atoi(argv[3]));
+    if (argc > 5) solver.setRecordModel(true, std::atoi(argv[5]), argv[4]);
     auto U = solver.solve();
     double s = 0.0;
     for (auto &row : U)
