@@ -27,22 +27,19 @@ FVM_HD void limit1_adj(double qL, double q, double qR, const Line &L, double sb,
     const double c = (qR - qL) * L.ih_c;
     const double l = (q - qL) * L.ih_a;
     const double r = (qR - q) * L.ih_a;
-    qLb = 0.0;
-    qb = 0.0;
-    qRb = 0.0;
-    if (c * r > 0.0 && l * r > 0.0) {
-        const double ac = fabs(c), al = fabs(l), ar = fabs(r);
-        if (ac <= al && ac <= ar) {
-            qRb = L.ih_c * sb;
-            qLb = -L.ih_c * sb;
-        } else if (al <= ar) {
-            qb = L.ih_a * sb;
-            qLb = -L.ih_a * sb;
-        } else {
-            qRb = L.ih_a * sb;
-            qb = -L.ih_a * sb;
-        }
-    }
+    // branch-free (selected weights): assigning the three outputs in
+    // separate branches makes the compiler keep them in private memory
+    const bool act = c * r > 0.0 && l * r > 0.0;
+    const double ac = fabs(c), al = fabs(l), ar = fabs(r);
+    const bool selc = ac <= al && ac <= ar;
+    const bool sell = !selc && al <= ar;
+    const bool selr = !selc && !sell;
+    const double wL = selc ? -L.ih_c : (sell ? -L.ih_a : 0.0);
+    const double wC = sell ? L.ih_a : (selr ? -L.ih_a : 0.0);
+    const double wR = selc ? L.ih_c : (selr ? L.ih_a : 0.0);
+    qLb = act ? wL * sb : 0.0;
+    qb = act ? wC * sb : 0.0;
+    qRb = act ? wR * sb : 0.0;
 }
 
 FVM_HD void slope_adj(const Q &qL, const Q &q, const Q &qR, const Line &L, const Q &sb, Q &qLb,
