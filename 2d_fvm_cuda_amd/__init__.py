@@ -67,7 +67,7 @@ EXPORTS = [
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
-    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe",
+    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe", "fvm2d_recip_probe",
     "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
     "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
@@ -120,6 +120,7 @@ def lib():
         "fvm2d_batch_destroy": (ctypes.c_int, [_vp]),
         "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
         "fvm2d_valu_probe": (ctypes.c_int, [ctypes.c_int, _dp]),
+        "fvm2d_recip_probe": (ctypes.c_int, [_dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -148,6 +149,14 @@ def valu_probe(kind: int) -> float:
     r = ctypes.c_double(0.0)
     check(lib().fvm2d_valu_probe(int(kind), ctypes.byref(r)), "fvm2d_valu_probe")
     return r.value
+
+
+def recip_probe():
+    """Max relative errors (raw rcp, refined rcp, raw rsq, refined rsq) of
+    the device reciprocals the FAST path uses."""
+    out = (ctypes.c_double * 4)()
+    check(lib().fvm2d_recip_probe(out), "fvm2d_recip_probe")
+    return tuple(out)
 
 
 def device_available() -> bool:

@@ -1,4 +1,7 @@
 // C ABI of include/fvm2d.h for the plain and batched forward solvers.
+#include <cmath>
+#include <vector>
+
 #include "fvm2d_engine.h"
 
 using namespace fvm2d;
@@ -428,6 +431,63 @@ int fvm2d_valu_probe(int kind, double *per_simd_per_s) {
         *per_simd_per_s = ops / (ms * 1e-3) / (4.0 * cus);
     }
     return rc;
+}
+
+}  // extern "C"
+
+// Accuracy of the fast path's reciprocal and reciprocal square root: raw
+// v_rcp_f64 / v_rsq_f64 and the refined rcp() / rsq() of fvm2d_cell.h.
+__global__ void recip_probe_kernel(const double *x, double *out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i];
+    out[4 * i + 0] = __builtin_amdgcn_rcp(a);
+    out[4 * i + 1] = rcp(a);
+    out[4 * i + 2] = __builtin_amdgcn_rsq(a);
+    out[4 * i + 3] = rsq(a);
+}
+
+extern "C" {
+
+int fvm2d_recip_probe(double *max_rel_err4) {
+    if (!max_rel_err4) return fail(FVM2D_ERR_ARG, "null output");
+    if (int rc0 = device_ok()) return rc0;
+    const int n = 1 << 18;
+    std::vector<double> x(n), o(4 * (size_t)n);
+    unsigned long long st = 0x9e3779b97f4a7c15ull;
+    for (int i = 0; i < n; i++) {  // 2^u, u uniform in [-8, 8) (splitmix64)
+        st += 0x9e3779b97f4a7c15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        x[i] = std::exp2(-8.0 + 16.0 * (double)(z >> 11) * 0x1.0p-53);
+    }
+    double *dx = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc((void **)&dx, n * sizeof(double)));
+    if (hipMalloc((void **)&dout, 4 * (size_t)n * sizeof(double)) != hipSuccess) {
+        hipFree(dx);
+        return fail(FVM2D_ERR_HIP, "hipMalloc failed");
+    }
+    int rc = FVM2D_OK;
+    if (hipMemcpy(dx, x.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(FVM2D_ERR_HIP, "copy failed");
+    } else {
+        recip_probe_kernel<<<(n + 255) / 256, 256>>>(dx, dout, n);
+        if (hipMemcpy(o.data(), dout, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(FVM2D_ERR_HIP, "recip probe failed");
+    }
+    hipFree(dx);
+    hipFree(dout);
+    if (rc) return rc;
+    for (int k = 0; k < 4; k++) max_rel_err4[k] = 0.0;
+    for (int i = 0; i < n; i++) {
+        const long double r = 1.0L / (long double)x[i], q = 1.0L / sqrtl((long double)x[i]);
+        const long double ref[4] = {r, r, q, q};
+        for (int k = 0; k < 4; k++)
+            max_rel_err4[k] = std::max(max_rel_err4[k], (double)fabsl((o[4 * (size_t)i + k] - ref[k]) / ref[k]));
+    }
+    return FVM2D_OK;
 }
 
 }  // extern "C"

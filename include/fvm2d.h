@@ -149,6 +149,13 @@ int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out);
  * kernel can reach on the card at its sustained clock. */
 int fvm2d_valu_probe(int kind, double *per_simd_per_s);
 
+/* Diagnostic with no counterpart in the reference: maximum relative error,
+ * over 2^18 arguments 2^u (u uniform in [-8, 8)), of the device's raw
+ * v_rcp_f64, the FAST path's refined reciprocal, raw v_rsq_f64 and the
+ * refined reciprocal square root (out[0..3]), against long-double host
+ * references. */
+int fvm2d_recip_probe(double *max_rel_err4);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- *
  * Hand-written reverse mode of the full split step (limiter and GRP
  * differentiated along the branch taken, dt differentiated through its

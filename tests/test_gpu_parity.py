@@ -170,3 +170,12 @@ def test_nan_state_reports_error(gpu, problems):
     U0[10, 10, 1] = -1.0  # negative pressure -> NaN sound speed
     with pytest.raises(gpu.FVM2DError, match="non-finite"):
         gpu.solve(U0, t_end=0.1)
+
+
+def test_fast_reciprocals_accuracy(gpu):
+    """The FAST path's one-Newton-step reciprocal and reciprocal square root
+    stay within a few ulps (the hardware estimates carry ~24 bits)."""
+    raw_rcp, rcp, raw_rsq, rsq = gpu.recip_probe()
+    print(f"rcp raw {raw_rcp:.3e} refined {rcp:.3e}; rsq raw {raw_rsq:.3e} refined {rsq:.3e}")
+    assert raw_rcp < 1e-6 and raw_rsq < 1e-6
+    assert rcp < 1e-14 and rsq < 1e-14
