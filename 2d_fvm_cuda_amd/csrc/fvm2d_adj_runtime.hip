@@ -272,10 +272,13 @@ struct AdjEngine {
         }
         HIPCHK(hipMemcpyAsync(d_adesc, had.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, fwd.stream));
         adj_launch_smax(d_adesc, n, fwd.stream);
-        adj_launch_fwd(d_adesc, d_tfwd, nfwd, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr2, nr2, 0, 0, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr3a, nr3a, 1, nr2, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr3b, nr3b, 2, nr2 + nr3a, fwd.stream);
+        // one wave per SIMD for the sweep adjoints (they need > 256
+        // registers), four for the lighter forward recompute
+        const int simds = 4 * fwd.cu_count;
+        adj_launch_fwd(d_adesc, d_tfwd, nfwd, 4 * simds, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr2, nr2, 0, 0, simds, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr3a, nr3a, 1, nr2, simds, fwd.stream);
+        adj_launch_sweep(d_adesc, d_tr3b, nr3b, 2, nr2 + nr3a, simds, fwd.stream);
         adj_launch_dt_term(d_adesc, n, nr2 + nr3a + nr3b, fwd.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(r1, fwd.stream));

@@ -105,16 +105,27 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-// fill^T: adjoint of the filled state U' at interior cell (j, c), plus the
-// adjoints of every ghost cell whose fill copies (j, c); ghost cells of the
-// averaged state receive nothing (the fill overwrites them).
-__device__ Q lamC_at(const AdjDesc &d, const double *lam, int j, int c) {
+// fill^T: adjoint of the filled state U' at interior cell (j, c) (zero for
+// ghost cells: the fill overwrites them), plus the adjoints of every ghost
+// cell whose fill copies (j, c).  Split in the plain load at the cell
+// (issued ahead with the task's other loads) and the ghost additions, which
+// only cells on the first/last two interior rows or columns have.
+__device__ __forceinline__ bool is_interior(const AdjDesc &d, int j, int c) {
+    return !(j < 2 || j > d.ny - 3 || c < 2 || c > d.nx - 3);
+}
+
+__device__ __forceinline__ Q lam_at(const AdjDesc &d, const double *lam, int j, int c) {
+    const long long P = d.plane, o = (long long)j * d.pitch + c;
+    const Q s{lam[o], lam[P + o], lam[2 * P + o], lam[3 * P + o]};
+    const bool in = is_interior(d, j, c);
+    return Q{in ? s.r : 0.0, in ? s.p : 0.0, in ? s.un : 0.0, in ? s.ut : 0.0};
+}
+
+__device__ Q lamC_ghosts(const AdjDesc &d, const double *lam, int j, int c, Q s) {
     const int nx = d.nx, ny = d.ny;
-    if (j < 2 || j > ny - 3 || c < 2 || c > nx - 3) return Q{0.0, 0.0, 0.0, 0.0};
-    const long long P = d.plane;
-    long long o = (long long)j * d.pitch + c;
-    Q s{lam[o], lam[P + o], lam[2 * P + o], lam[3 * P + o]};
+    if (!is_interior(d, j, c)) return s;
     if (j == 2 || j == ny - 4 || j == ny - 3 || c == 2 || c == nx - 3) {
+        const long long P = d.plane;
         const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
         const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
 #pragma unroll
@@ -126,7 +137,7 @@ __device__ Q lamC_at(const AdjDesc &d, const double *lam, int j, int c) {
                 if (!(gj < 2 || gj > ny - 3 || gi < 2 || gi > nx - 3)) continue;
                 const GhostSrc g = ghost_source(d.strategy, gj, gi, nx, ny, 0.0);
                 if (g.is_const || g.sj != j || g.si != c) continue;
-                o = (long long)gj * d.pitch + gi;
+                const long long o = (long long)gj * d.pitch + gi;
                 s.r += lam[o];
                 s.p += lam[P + o];
                 s.un += lam[2 * P + o];
@@ -137,24 +148,50 @@ __device__ Q lamC_at(const AdjDesc &d, const double *lam, int j, int c) {
     return s;
 }
 
+// The task a wave works on is wave-uniform: keep it in scalar registers.
+__device__ __forceinline__ int4 uniform_task(const int4 *__restrict__ tasks, int tk) {
+    const int4 v = tasks[tk];
+    return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                     __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
 // ---------------------------------------------------------------- R1
-// Forward first sweeps A1 = X(U) (X tasks) and B1 = Y(U) (Y tasks); X tasks
-// also reduce (s_max, first arg-max index) of their owned cells.
+// Forward first sweeps A1 = X(U) (X tasks) and B1 = Y(U) (Y tasks).  Each
+// wave walks a strided list of tasks and loads the next task's line while it
+// computes the current one.
 __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ descs,
                                                       const int4 *__restrict__ tasks, int ntasks) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int tk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int nwv = gridDim.x * (blockDim.x / kWave);
+    int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
     if (tk >= ntasks) return;
-    const int4 t = tasks[tk];
-    AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
-    const LineMap m = line_map(d, t, lane, kFwdHalo, kFwdOwn);
-    const Line L = line_of(d, m);
-    const double dt = d.ctl->dt;
-    const Q q = load_line(d.U, d.plane, m);
-    const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
-    const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
-    const bool upd = m.k >= 2 && m.k <= m.n - 3;
-    if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
+    int4 t = uniform_task(tasks, tk);
+    LineMap m = line_map(descs[t.x], t, lane, kFwdHalo, kFwdOwn);
+    Q q = load_line(descs[t.x].U, descs[t.x].plane, m);
+    for (;;) {
+        const int tn = tk + nwv;
+        const bool more = tn < ntasks;
+        int4 t2 = t;
+        LineMap m2 = m;
+        Q q2 = q;
+        if (more) {
+            t2 = uniform_task(tasks, tn);
+            m2 = line_map(descs[t2.x], t2, lane, kFwdHalo, kFwdOwn);
+            q2 = load_line(descs[t2.x].U, descs[t2.x].plane, m2);
+        }
+        const AdjDesc &d = descs[t.x];
+        const Line L = line_of(d, m);
+        const double dt = d.ctl->dt;
+        const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
+        const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
+        const bool upd = m.k >= 2 && m.k <= m.n - 3;
+        if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
+        if (!more) break;
+        tk = tn;
+        t = t2;
+        m = m2;
+        q = q2;
+    }
 }
 
 constexpr int kSmaxBlocks = 256;  // blocks per instance of the s_max kernel
@@ -233,35 +270,45 @@ __global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ des
 // lam_A1 / lam_B1.  mode 1 (R3a, X tasks): input U, output adjoint lam_A1,
 // result written to lam_out.  mode 2 (R3b, Y tasks): input U, output adjoint
 // lam_B1, result added to lam_out.
-__global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ descs,
-                                                        const int4 *__restrict__ tasks, int ntasks,
-                                                        int mode, int part_base) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int tk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (tk >= ntasks) return;
-    const int4 t = tasks[tk];
-    AdjDesc &d = descs[__builtin_amdgcn_readfirstlane(t.x)];
-    const LineMap m = line_map(d, t, lane, kAdjHalo, kAdjOwn);
+
+// The loads of one task, issued one task ahead: the input state at the
+// lane's cell, the output adjoint (mode 0: lam' at the cell; the fill^T
+// ghost additions come at use), and in mode 2 the value the result is
+// added to.
+struct SweepLoads {
+    Q q, qbo, acc;
+};
+
+__device__ __forceinline__ SweepLoads sweep_loads(const AdjDesc &d, const LineMap &m, int mode) {
+    const long long P = d.plane;
+    SweepLoads in;
+    in.q = load_line(mode == 0 ? (m.ydir ? d.A1 : d.B1) : d.U, P, m);
+    if (mode == 0) {
+        const long long o = off_of(m);
+        in.qbo = lam_at(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch));
+    } else {
+        in.qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
+    }
+    in.acc = mode == 2 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
+    return in;
+}
+
+__device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, const SweepLoads &in,
+                                           int lane, int mode, int part) {
     const Line L = line_of(d, m);
     const Gas &G = d.gas;
     const double dt = d.ctl->dt;
     const double lam = dt / L.h;
     const long long P = d.plane;
 
-    const double *qin;
-    Q qbo;  // adjoint of this sweep's output at this lane's cell
+    Q qbo = in.qbo;
     if (mode == 0) {
-        qin = m.ydir ? d.A1 : d.B1;
         const long long o = off_of(m);
-        const int j = (int)(o / d.pitch), c = (int)(o % d.pitch);
-        const Q s = lamC_at(d, d.lam_in, j, c);
+        const Q s = lamC_ghosts(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch), qbo);
         const Q h{0.5 * s.r, 0.5 * s.p, 0.5 * s.un, 0.5 * s.ut};
         qbo = pick(m.ydir, swap_uv(h), h);
-    } else {
-        qin = d.U;
-        qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
     }
-    const Q q = load_line(qin, P, m);
+    const Q q = in.q;
 
     const bool inl = m.k >= 0 && m.k < m.n;
     const bool upd = inl && m.k >= 2 && m.k <= m.n - 3;
@@ -311,13 +358,47 @@ __global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ de
     slope_adj(qL, q, qR, L, st, pL, pC, pR);
     const Q fromR = shl(pL), fromL = shr(pR);
     const Q cb = pick(upd, cons_adj(q, Ub, G), qbo);
-    qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r, qb.p + pC.p + fromR.p + fromL.p + cb.p,
-           qb.un + pC.un + fromR.un + fromL.un + cb.un, qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut};
+    qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r + in.acc.r, qb.p + pC.p + fromR.p + fromL.p + cb.p + in.acc.p,
+           qb.un + pC.un + fromR.un + fromL.un + cb.un + in.acc.un,
+           qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut + in.acc.ut};
 
     double *out = mode == 0 ? (m.ydir ? d.lamA1 : d.lamB1) : d.lam_out;
-    if (own) store_line(out, P, m, qb, mode == 2);
-    const double part = wave_sum(own ? dtb : 0.0);
-    if (lane == 0) d.dtpart[part_base + tk] = part;
+    if (own) store_line(out, P, m, qb, false);
+    const double part_sum = wave_sum(own ? dtb : 0.0);
+    if (lane == 0) d.dtpart[part] = part_sum;
+}
+
+// Each wave walks a strided list of tasks and issues the next task's loads
+// before it computes the current one (one wave per SIMD fits: the sweep
+// adjoint needs more than 256 registers).
+__global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ descs,
+                                                        const int4 *__restrict__ tasks, int ntasks,
+                                                        int mode, int part_base) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nwv = gridDim.x * (blockDim.x / kWave);
+    int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+    if (tk >= ntasks) return;
+    int4 t = uniform_task(tasks, tk);
+    LineMap m = line_map(descs[t.x], t, lane, kAdjHalo, kAdjOwn);
+    SweepLoads in = sweep_loads(descs[t.x], m, mode);
+    for (;;) {
+        const int tn = tk + nwv;
+        const bool more = tn < ntasks;
+        int4 t2 = t;
+        LineMap m2 = m;
+        SweepLoads in2 = in;
+        if (more) {
+            t2 = uniform_task(tasks, tn);
+            m2 = line_map(descs[t2.x], t2, lane, kAdjHalo, kAdjOwn);
+            in2 = sweep_loads(descs[t2.x], m2, mode);
+        }
+        sweep_task(descs[t.x], m, in, lane, mode, part_base + tk);
+        if (!more) break;
+        tk = tn;
+        t = t2;
+        m = m2;
+        in = in2;
+    }
 }
 
 // ---------------------------------------------------------------- R4
@@ -434,16 +515,18 @@ __global__ void prim_grad_to_cons_kernel(const double *__restrict__ cons, const 
 // ---------------------------------------------------------------- launchers
 static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
-void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s) {
-    hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks, ntasks);
+void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves, hipStream_t s) {
+    const int waves = ntasks < max_waves ? ntasks : max_waves;
+    hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(waves, 4)), dim3(256), 0, s, descs, tasks, ntasks);
 }
 void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
     hipLaunchKernelGGL(adj_smax_kernel, dim3(kSmaxBlocks, ninst), dim3(256), 0, s, descs);
 }
 int adj_smax_pairs() { return kSmaxBlocks; }
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(adj_sweep_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs, tasks,
+                      int max_waves, hipStream_t s) {
+    const int waves = ntasks < max_waves ? ntasks : max_waves;
+    hipLaunchKernelGGL(adj_sweep_kernel, dim3(blocks_for(waves, 4)), dim3(256), 0, s, descs, tasks,
                        ntasks, mode, part_base);
 }
 void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s) {

@@ -37,13 +37,14 @@ struct AdjDesc {
     int part_lo[3], part_hi[3];    // this instance's ranges in dtpart
 };
 
-void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s);
+// max_waves: the waves of a launch (each walks a strided task list).
+void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves, hipStream_t s);
 // s_max, its arg-max and dt of every instance (one launch); the pair arrays
 // hold adj_smax_pairs() entries per instance.
 void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s);
 int adj_smax_pairs();
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
-                      hipStream_t s);
+                      int max_waves, hipStream_t s);
 void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s);
 void adj_launch_cons_to_prim(const double *cons, double *prim, int nx, int ny, int pitch,
                              long long plane, double g1, hipStream_t s);

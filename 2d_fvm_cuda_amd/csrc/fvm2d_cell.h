@@ -443,7 +443,11 @@ FVM_HD double speed(double rho, double p, double u, double v, const Gas &G) {
         double velo = sqrt(u * u + v * v), c = sqrt(G.gamma * p / rho);
         return fabs(velo) + c;
     }
-    return sqrt(fma(u, u, v * v)) + sqrt(G.gamma * p * rcp(rho));
+    // a = sqrt(gamma p / rho) = gamma p / sqrt(gamma p rho); |(u, v)| = q / sqrt(q)
+    // (below 1e-300, q contributes nothing at double precision next to a)
+    const double gp = G.gamma * p, q = fma(u, u, v * v);
+    const double vel = q > 1e-300 ? q * rsq(q) : 0.0;
+    return vel + gp * rsq(gp * rho);
 }
 
 }  // namespace fvm2d
