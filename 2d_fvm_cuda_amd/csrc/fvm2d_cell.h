@@ -139,6 +139,63 @@ FVM_HD Q face_plus(const Q &q, const Q &s, const Line &L) {
              fma(s.ut, L.hh, q.ut)};
 }
 
+// Fast-path slopes in cell units, h sigma: the limiter without its final
+// 1/h, and face states q -/+ (h sigma)/2.  The GRP's time derivatives are
+// linear in the slopes, so with h sigma in place of sigma they come out
+// multiplied by h, and the flux takes k/h in place of k: one multiplication
+// per slope component less.
+FVM_HD double limit1_h(double qL, double q, double qR, double alpha) {
+    const double dL = q - qL, dR = qR - q, dC = qR - qL;
+    const double sa = copysign(alpha, dC);
+    const double m = fmax(0.0, fmin(fmin(sa * dL, sa * dR), 0.5 * fabs(dC)));
+    return copysign(m, dC);
+}
+
+FVM_HD Q slope_h(const Q &qL, const Q &q, const Q &qR, double alpha) {
+    return Q{limit1_h(qL.r, q.r, qR.r, alpha), limit1_h(qL.p, q.p, qR.p, alpha),
+             limit1_h(qL.un, q.un, qR.un, alpha), limit1_h(qL.ut, q.ut, qR.ut, alpha)};
+}
+
+FVM_HD Q face_minus_h(const Q &q, const Q &hs) {
+    return Q{fma(-0.5, hs.r, q.r), fma(-0.5, hs.p, q.p), fma(-0.5, hs.un, q.un),
+             fma(-0.5, hs.ut, q.ut)};
+}
+
+FVM_HD Q face_plus_h(const Q &q, const Q &hs) {
+    return Q{fma(0.5, hs.r, q.r), fma(0.5, hs.p, q.p), fma(0.5, hs.un, q.un), fma(0.5, hs.ut, q.ut)};
+}
+
+// Slopes, face states and the flux's time step of the step kernel's
+// sweeps: the reference's forms in EXACT mode, cell units in FAST mode.
+template <bool Exact>
+FVM_HD Q sweep_slope(const Q &qL, const Q &q, const Q &qR, const Line &L) {
+    if constexpr (Exact)
+        return slope<true>(qL, q, qR, L);
+    else
+        return slope_h(qL, q, qR, L.alpha);
+}
+
+template <bool Exact>
+FVM_HD Q sweep_face_minus(const Q &q, const Q &s, const Line &L) {
+    if constexpr (Exact)
+        return face_minus<true>(q, s, L);
+    else
+        return face_minus_h(q, s);
+}
+
+template <bool Exact>
+FVM_HD Q sweep_face_plus(const Q &q, const Q &s, const Line &L) {
+    if constexpr (Exact)
+        return face_plus<true>(q, s, L);
+    else
+        return face_plus_h(q, s);
+}
+
+template <bool Exact>
+FVM_HD double sweep_k(double dt, const Line &L) {
+    return Exact ? dt : dt * L.ih;
+}
+
 // ---------------------------------------------------------------- GRP
 // Relaxation GRP at one interface, src/basicFuns.cuh:86-189.
 //   fl, fr : left/right face states; sl, sr : slopes of the left/right cells.
@@ -297,11 +354,6 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double iSl = rsq(xl), iSr = rsq(xr);
     const double Sl = xl * iSl, Sr = xr * iSr;
     const double irl = gpl * iSl * iSl, irr = gpr * iSr * iSr;
-    const double prl = pl * irl, prr = pr * irr;
-    const double el = prl * g1i, er = prr * g1i;
-    // e_x = (p_x - p rho_x / rho) / rho / (gamma - 1)
-    const double elx = fma(-prl, sl.r, sl.p) * (irl * g1i);
-    const double erx = fma(-prr, sr.r, sr.p) * (irr * g1i);
     // c_x = 0.5 sqrt(gamma/(p rho)) (p rho_x + rho p_x) = 0.5 gamma/S (...)
     const double clx = (0.5 * gamma) * iSl * fma(pl, sl.r, rl * sl.p);
     const double crx = (0.5 * gamma) * iSr * fma(pr, sr.r, rr * sr.p);
@@ -331,8 +383,10 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
         // (measured: 13% less time per step than evaluating both and selecting).
         const double rK = supL ? rl : rr, pK = supL ? pl : pr, u = supL ? ul : ur;
         const double irK = supL ? irl : irr, cK = supL ? cl : cr;
-        const double eK = supL ? el : er, eKx = supL ? elx : erx;
         const Q sK = pick(supL, sl, sr);
+        // e = p / rho / (gamma - 1), e_x = (p_x - p rho_x / rho) / rho / (gamma - 1)
+        const double prK = pK * irK;
+        const double eK = prK * g1i, eKx = fma(-prK, sK.r, sK.p) * (irK * g1i);
         W = IfState{rK, pK, u, supL ? fl.ut : fr.ut, eK};
         Wt.r = -fma(rK, sK.un, u * sK.r);
         Wt.un = -fma(u, sK.un, sK.p * irK);
@@ -351,22 +405,27 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double cK = right ? cr : cl, icK = right ? icr : icl;
     const double irK = right ? irr : irl;
     const double uK = right ? ur : ul, pK = right ? pr : pl;
-    const double eK = right ? er : el, eKx = right ? erx : elx, cKx = right ? crx : clx;
+    const double cKx = right ? crx : clx;
     const Q sK = pick(right, sr, sl);
+    // e and e_x of side K only (the star state needs no other)
+    const double prK = pK * irK;
+    const double eK = prK * g1i, eKx = fma(-prK, sK.r, sK.p) * (irK * g1i);
     // k and d use u = 0 (see the exact form)
     const double kl = fma(-0.5 * ul, clx, fma(-cl, sl.un, -sl.p));
     const double kr = fma(0.5 * ur, crx, fma(cr, sr.un, -sr.p));
     const double dl = cl * irl * fma(-2.0 * cl, sl.un, fma(-ul, clx, -2.0 * sl.p));
     const double dr = cr * irr * fma(2.0 * cr, sr.un, fma(ur, crx, -2.0 * sr.p));
     const double Dp = fma(cr, dl, -cl * dr) * 0.5 * ics;
-    const double den = fma(s, um - uK, cK * irK);
+    const double cirK = cK * irK;
+    const double den = fma(s, um - uK, cirK);  // c_K / rho*
     const double rho = cK * rcp(den);
-    const double irho = den * icK;
     const double icK2 = icK * icK;
     const double e = fma((pm - pK) * (pm + pK), 0.5 * icK2, eK);
     const double r3 = rho * rho * rho;
-    const double t1 = fma(cK * irK * irK, cK * sK.r, fma(s * cK, sK.un, 1.5 * (um - uK) * cKx));
-    const double rho_t = fma(-um * r3 * irK * icK2, t1, r3 * icK2 * icK * fma(cK, irho, -s * um) * Dp);
+    const double t1 = fma(cirK * irK, cK * sK.r, fma(s * cK, sK.un, 1.5 * (um - uK) * cKx));
+    // the reference's (c_K / rho* - s u*) with c_K / rho* = den = s (u* - u_K)
+    // + c_K / rho_K reduces to c_K / rho_K - s u_K
+    const double rho_t = fma(-um * r3 * irK * icK2, t1, r3 * icK2 * icK * fma(-s, uK, cirK) * Dp);
     const double urho = um * rho;
     const double A = cl * irl * fma(urho, icr, 1.0) * kl;
     const double B = cr * irr * fma(-urho, icl, 1.0) * kr;

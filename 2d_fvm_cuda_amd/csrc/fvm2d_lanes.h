@@ -38,13 +38,13 @@ template <bool Exact>
 __device__ __forceinline__ Q xsweep(const Q &q, const Line &L, const Gas &G, double dt,
                                     double lam) {
     const Q qL = shr(q), qR = shl(q);
-    const Q s = slope<Exact>(qL, q, qR, L);
-    const Q fm = face_minus<Exact>(q, s, L);
-    const Q fp = face_plus<Exact>(q, s, L);
+    const Q s = sweep_slope<Exact>(qL, q, qR, L);
+    const Q fm = sweep_face_minus<Exact>(q, s, L);
+    const Q fp = sweep_face_plus<Exact>(q, s, L);
     const Q fpL = shr(fp), sL = shr(s);
     IfState W, Wt;
     grp<Exact>(fpL, fm, sL, s, G, W, Wt);  // interface on the west face
-    const Q F = flux<Exact>(W, Wt, dt);
+    const Q F = flux<Exact>(W, Wt, sweep_k<Exact>(dt, L));
     const Q FR = shl(F);                   // interface on the east face
     return update<Exact>(q, F, FR, lam, G);
 }

@@ -38,6 +38,10 @@ def _cases(problems):
     cases.append(("rp70x130", problems.set_rp_initial_data(
         [1.0, 1.0, 0.75, -0.5, 0], [2.0, 1.0, 0.75, 0.5, 0], [1.0, 1.0, -0.75, 0.5, 0],
         [3.0, 1.0, -0.75, -0.5, 0], 70, 130), dict(t_end=0.02, alpha=1.7, cfl=0.4)))
+    # pressure ratio 1e18 (test2D.cu:118-133 on [0,8]^2): supersonic
+    # interfaces on both sides of the blast
+    cases.append(("sedov34", problems.initialize_Sedov1D(34), dict(t_end=1.0, max_steps=10, width=8.0,
+                                                                     height=8.0)))
     return cases
 
 
@@ -46,7 +50,7 @@ def cases(problems):
     return _cases(problems)
 
 
-@pytest.mark.parametrize("idx", range(10))
+@pytest.mark.parametrize("idx", range(11))
 def test_exact_mode_bitwise(gpu, orc, cases, idx):
     name, U0, prm = cases[idx]
     print(f"case {name} shape {U0.shape} {prm}", flush=True)
@@ -57,7 +61,7 @@ def test_exact_mode_bitwise(gpu, orc, cases, idx):
     assert diff.size == 0, f"{name}: {len(diff)} differing values, first {diff[:5].tolist()}"
 
 
-@pytest.mark.parametrize("idx", range(10))
+@pytest.mark.parametrize("idx", range(11))
 def test_fast_mode_parity(gpu, orc, cases, idx):
     name, U0, prm = cases[idx]
     print(f"case {name} shape {U0.shape} {prm}", flush=True)
