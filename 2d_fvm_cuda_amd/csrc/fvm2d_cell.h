@@ -410,30 +410,36 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     // e and e_x of side K only (the star state needs no other)
     const double prK = pK * irK;
     const double eK = prK * g1i, eKx = fma(-prK, sK.r, sK.p) * (irK * g1i);
-    // k and d use u = 0 (see the exact form)
+    // k_l, k_r use u = 0 (see the exact form).  The reference's d_K equal
+    // 2 (c_K / rho_K) k_K, so with P = k_l / rho_l, R = k_r / rho_r and
+    // M = (c_l P + c_r R) / (c_l + c_r) its three time derivatives of both
+    // star sub-cases (src/basicFuns.cuh:151-185) reduce to
+    //   Dpi/Dt = c_l c_r (P - R) / (c_l + c_r),
+    //   p_t = Dpi/Dt + u* rho* M,   u_t = M + u* rho* Dpi/Dt / c_K^2.
     const double kl = fma(-0.5 * ul, clx, fma(-cl, sl.un, -sl.p));
     const double kr = fma(0.5 * ur, crx, fma(cr, sr.un, -sr.p));
-    const double dl = cl * irl * fma(-2.0 * cl, sl.un, fma(-ul, clx, -2.0 * sl.p));
-    const double dr = cr * irr * fma(2.0 * cr, sr.un, fma(ur, crx, -2.0 * sr.p));
-    const double Dp = fma(cr, dl, -cl * dr) * 0.5 * ics;
+    const double P = kl * irl, R = kr * irr;
+    const double Dp = (cl * cr) * ics * (P - R);
+    const double M = fma(cl, P, cr * R) * ics;
     const double cirK = cK * irK;
     const double den = fma(s, um - uK, cirK);  // c_K / rho*
-    const double rho = cK * rcp(den);
+    const double iden = rcp(den);
+    const double rho = cK * iden;
     const double icK2 = icK * icK;
-    const double e = fma((pm - pK) * (pm + pK), 0.5 * icK2, eK);
-    const double r3 = rho * rho * rho;
+    // e* = e_K + (p*^2 - p_K^2) / (2 c_K^2)
+    const double D2 = (pm - pK) * (pm + pK) * icK2;
+    const double e = fma(0.5, D2, eK);
     const double t1 = fma(cirK * irK, cK * sK.r, fma(s * cK, sK.un, 1.5 * (um - uK) * cKx));
-    // the reference's (c_K / rho* - s u*) with c_K / rho* = den = s (u* - u_K)
-    // + c_K / rho_K reduces to c_K / rho_K - s u_K
-    const double rho_t = fma(-um * r3 * irK * icK2, t1, r3 * icK2 * icK * fma(-s, uK, cirK) * Dp);
+    // rho*^3 / (rho_K c_K^2) = (c_K / rho_K) / den^3 and rho*^3 / c_K^3 =
+    // 1 / den^3; the reference's (c_K / rho* - s u*), with c_K / rho* = den
+    // = s (u* - u_K) + c_K / rho_K, reduces to c_K / rho_K - s u_K
+    const double iden3 = iden * iden * iden;
+    const double rho_t = iden3 * fma(-um * cirK, t1, fma(-s, uK, cirK) * Dp);
     const double urho = um * rho;
-    const double A = cl * irl * fma(urho, icr, 1.0) * kl;
-    const double B = cr * irr * fma(-urho, icl, 1.0) * kr;
-    const double p_t = ics * (cr * A - cl * B);
-    const double a1 = right ? icr : cr * icl * icl;
-    const double a2 = right ? cl * icr * icr : icl;
-    const double u_t = ics * fma(cl * irl * fma(urho, a1, 1.0), kl, cr * irr * fma(-urho, a2, 1.0) * kr);
-    const double e_t = fma(-urho * irK, fma(2.0 * (eK - e) * icK, cKx, fma(-pK * icK2, sK.p, eKx)),
+    const double p_t = fma(urho, M, Dp);
+    const double u_t = fma(urho * icK2, Dp, M);
+    // 2 (e_K - e*) / c_K = -D2 / c_K
+    const double e_t = fma(-urho * irK, fma(-D2 * icK, cKx, fma(-pK * icK2, sK.p, eKx)),
                            pm * icK2 * p_t);
     W = IfState{rho, pm, um, right ? fr.ut : fl.ut, e};
     Wt = IfState{rho_t, p_t, u_t, -um * (rho * irK * sK.ut), e_t};
