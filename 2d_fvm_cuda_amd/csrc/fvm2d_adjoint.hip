@@ -93,6 +93,7 @@ __device__ __forceinline__ Line line_of(const AdjDesc &d, const LineMap &m) {
     L.ih_c = m.ydir ? d.ly.ih_c : d.lx.ih_c;
     L.hh = m.ydir ? d.ly.hh : d.lx.hh;
     L.alpha = m.ydir ? d.ly.alpha : d.lx.alpha;
+    L.quarter = d.lx.quarter;
     return L;
 }
 
