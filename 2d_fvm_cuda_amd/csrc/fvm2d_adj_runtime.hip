@@ -28,7 +28,6 @@ struct AdjEngine {
     std::vector<unsigned long long *> pmax;
     std::vector<long long *> pidx;
     double *dtpart = nullptr;
-    AdjDesc *d_adesc = nullptr;
     AdjCtl *d_actl = nullptr;
     std::vector<AdjDesc> had;
     int4 *d_tfwd = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
@@ -38,7 +37,8 @@ struct AdjEngine {
     // schedule
     std::vector<std::vector<long long>> cost;
     std::vector<std::vector<int>> split;
-    std::vector<std::vector<double>> slot_t;  // [slot][inst] time of the stored state
+    double *d_slot_t = nullptr;      // [slot * n + inst] time of the stored state
+    AdjDesc *d_atab = nullptr;       // reverse-step descriptors, see build_desc_table
     std::vector<int> slot_step;      // step index held by each slot (-1: free)
     std::map<int, int> stored;       // step -> slot
     // stats
@@ -47,6 +47,11 @@ struct AdjEngine {
     int reverse_steps = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0, rev_ms_sum = 0.0;
     hipEvent_t r0 = nullptr, r1 = nullptr;
+    // reverse-step timing: a ring of event pairs read back lazily, so the
+    // host never waits for a reverse step before queueing the next one
+    static constexpr int kRing = 64;
+    hipEvent_t ring0[kRing] = {}, ring1[kRing] = {};
+    int ring_pending = 0;  // recorded pairs not yet added to rev_ms_sum
     bool have_forward = false;
     bool launch_chosen = false;  // the forward engine's launch geometry is fixed
 
@@ -54,6 +59,10 @@ struct AdjEngine {
         for (void *p : allocs) hipFree(p);
         if (r0) hipEventDestroy(r0);
         if (r1) hipEventDestroy(r1);
+        for (int k = 0; k < kRing; k++) {
+            if (ring0[k]) hipEventDestroy(ring0[k]);
+            if (ring1[k]) hipEventDestroy(ring1[k]);
+        }
     }
 
     template <class T>
@@ -81,6 +90,10 @@ struct AdjEngine {
         for (int i = 0; i < count; i++) max_cells = std::max(max_cells, (long long)nx[i] * ny[i]);
         HIPCHK(hipEventCreate(&r0));
         HIPCHK(hipEventCreate(&r1));
+        for (int k = 0; k < kRing; k++) {
+            HIPCHK(hipEventCreate(&ring0[k]));
+            HIPCHK(hipEventCreate(&ring1[k]));
+        }
         had.resize(n);
         ck.resize((size_t)n * slots);
         A1.resize(n), B1.resize(n), lamA1.resize(n), lamB1.resize(n), lamX.resize(n), lamY.resize(n);
@@ -122,7 +135,8 @@ struct AdjEngine {
         }
         rc = build_tasks();
         if (rc) return rc;
-        rc = dalloc(&d_adesc, n);
+        rc = dalloc(&d_slot_t, (size_t)slots * n);
+        if (!rc) rc = dalloc(&d_atab, (size_t)(slots + 2) * 2 * n);
         if (rc) return rc;
         build_schedule();
         HIPCHK(hipStreamSynchronize(fwd.stream));
@@ -212,45 +226,80 @@ struct AdjEngine {
         }
     }
 
-    // Copy the state of every instance from slot src (or the forward
-    // engine's buf0 when src < 0) to slot dst (or buf0 when dst < 0).
+    // State locations: a slot index s >= 0, or one of the forward engine's
+    // two time-loop buffers, kBuf0 (buf0) and kBuf1 (buf1).
+    static constexpr int kBuf0 = -1, kBuf1 = -2;
+
+    const double *state_ptr(int i, int where) const {
+        const InstDesc &d = fwd.hd[i];
+        return where == kBuf0 ? d.buf0 : where == kBuf1 ? d.buf1 : ck[(size_t)i * slots + where];
+    }
+
+    // Copy the state of every instance from src to dst (locations as above).
     int copy_state(int src, int dst) {
         for (int i = 0; i < n; i++) {
             const InstDesc &d = fwd.hd[i];
-            const double *a = src < 0 ? d.buf0 : ck[(size_t)i * slots + src];
-            double *b = dst < 0 ? d.buf0 : ck[(size_t)i * slots + dst];
-            HIPCHK(hipMemcpyAsync(b, a, 4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice, fwd.stream));
+            HIPCHK(hipMemcpyAsync(const_cast<double *>(state_ptr(i, dst)), state_ptr(i, src),
+                                  4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice, fwd.stream));
         }
         return FVM2D_OK;
     }
 
-    // Advance every instance by m steps from the state in slot src (times
-    // t_src per instance); the result is left in the forward engine's buf0
-    // and its times in t_out.
-    int advance(int src, int m, const std::vector<double> &t_src, std::vector<double> *t_out) {
-        int rc = copy_state(src, -1);
+    // Advance every instance by m >= 1 steps from the state in slot src,
+    // whose times are d_slot_t[src]; *res receives the engine buffer that
+    // holds the result (kBuf0 or kBuf1: the time loop alternates them) and
+    // the engine's control blocks its times.  Everything is queued on the
+    // stream: the host does not wait.
+    int advance(int src, int m, int *res) {
+        int rc = copy_state(src, kBuf0);
+        if (!rc) rc = fwd.begin_device(d_slot_t + (size_t)src * n);
+        if (!rc) rc = fwd.enqueue(m);
         if (rc) return rc;
-        rc = fwd.begin(t_src.data());
-        if (rc) return rc;
-        if (m > 0) {
-            rc = fwd.enqueue(m);
-            if (rc) return rc;
-        }
-        rc = fwd.read_ctl();
-        if (rc) return rc;
-        for (int i = 0; i < n; i++) {
-            if (fwd.h_ctl[i].status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite state in the forward run");
-            if (m & 1) std::swap(fwd.hd[i].buf0, fwd.hd[i].buf1);
-        }
-        if (m & 1) {
-            rc = fwd.upload_desc();
-            if (rc) return rc;
-        }
-        fwd.begun = false;  // the state is back in buf0
+        fwd.begun = false;
         recompute_steps += m;
-        if (t_out) {
-            t_out->resize(n);
-            for (int i = 0; i < n; i++) (*t_out)[i] = fwd.h_ctl[i].t;
+        *res = (m & 1) ? kBuf1 : kBuf0;
+        return FVM2D_OK;
+    }
+
+    // Store the state at location src into slot s together with its times
+    // (the engine's control blocks after an advance).
+    int store_slot(int src, int s) {
+        int rc = copy_state(src, s);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy2DAsync(d_slot_t + (size_t)s * n, sizeof(double), &fwd.d_ctl[0].t, sizeof(Ctl),
+                                sizeof(double), n, hipMemcpyDeviceToDevice, fwd.stream));
+        return FVM2D_OK;
+    }
+
+    // Descriptors of every reverse step, built once per backward call: for
+    // each state location (the slots, then buf0 and buf1) and each parity of
+    // the adjoint buffers (lam_in = lamX, lam_out = lamY, then swapped), the
+    // n instance descriptors.
+    int build_desc_table() {
+        std::vector<AdjDesc> tab((size_t)(slots + 2) * 2 * n);
+        for (int loc = 0; loc < slots + 2; loc++)
+            for (int par = 0; par < 2; par++)
+                for (int i = 0; i < n; i++) {
+                    AdjDesc a = had[i];
+                    a.U = state_ptr(i, loc < slots ? loc : (loc == slots ? kBuf0 : kBuf1));
+                    a.lam_in = par ? lamY[i] : lamX[i];
+                    a.lam_out = par ? lamX[i] : lamY[i];
+                    tab[((size_t)loc * 2 + par) * n + i] = a;
+                }
+        HIPCHK(hipMemcpy(d_atab, tab.data(), tab.size() * sizeof(AdjDesc), hipMemcpyHostToDevice));
+        return FVM2D_OK;
+    }
+
+    // Add the elapsed times of the oldest pending reverse steps to
+    // rev_ms_sum until at most keep remain pending.
+    int drain_ring(int keep) {
+        while (ring_pending > keep) {
+            const int k = (reverse_steps - ring_pending) % kRing;
+            HIPCHK(hipEventSynchronize(ring1[k]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ring0[k], ring1[k]));
+            rev_ms_sum += ms;
+            ring_pending--;
         }
         return FVM2D_OK;
     }
@@ -261,31 +310,29 @@ struct AdjEngine {
         return -1;
     }
 
-    // Reverse step n (U_n in slot s, or in buf0 when s < 0): lamX (dL/dU_{n+1})
-    // -> lamY (dL/dU_n); then the two are swapped.
+    // Reverse step n (U_n at location s): lamX (dL/dU_{n+1}) -> lamY
+    // (dL/dU_n); then the two are swapped.  The descriptors come from the
+    // table (the parity of the reverse steps done so far selects the
+    // adjoint buffers, which started as lamX -> lamY).
     int reverse_step(int s) {
-        HIPCHK(hipEventRecord(r0, fwd.stream));
-        for (int i = 0; i < n; i++) {
-            had[i].U = s < 0 ? fwd.hd[i].buf0 : ck[(size_t)i * slots + s];
-            had[i].lam_in = lamX[i];
-            had[i].lam_out = lamY[i];
-        }
-        HIPCHK(hipMemcpyAsync(d_adesc, had.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, fwd.stream));
-        adj_launch_smax(d_adesc, n, fwd.stream);
+        int rc = drain_ring(kRing - 1);
+        if (rc) return rc;
+        const int k = reverse_steps % kRing;
+        const int loc = s >= 0 ? s : (s == kBuf0 ? slots : slots + 1);
+        AdjDesc *descs = d_atab + ((size_t)loc * 2 + (reverse_steps & 1)) * n;
+        HIPCHK(hipEventRecord(ring0[k], fwd.stream));
+        adj_launch_smax(descs, n, fwd.stream);
         // one wave per SIMD for the sweep adjoints (they need > 256
         // registers), four for the lighter forward recompute
         const int simds = 4 * fwd.cu_count;
-        adj_launch_fwd(d_adesc, d_tfwd, nfwd, 4 * simds, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr2, nr2, 0, 0, simds, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr3a, nr3a, 1, nr2, simds, fwd.stream);
-        adj_launch_sweep(d_adesc, d_tr3b, nr3b, 2, nr2 + nr3a, simds, fwd.stream);
-        adj_launch_dt_term(d_adesc, n, nr2 + nr3a + nr3b, fwd.stream);
+        adj_launch_fwd(descs, d_tfwd, nfwd, 4 * simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr2, nr2, 0, 0, simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr3b, nr3b, 2, nr2 + nr3a, simds, fwd.stream);
+        adj_launch_dt_term(descs, n, nr2 + nr3a + nr3b, fwd.stream);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(r1, fwd.stream));
-        HIPCHK(hipEventSynchronize(r1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, r0, r1));
-        rev_ms_sum += ms;
+        HIPCHK(hipEventRecord(ring1[k], fwd.stream));
+        ring_pending++;
         reverse_steps++;
         std::swap(lamX, lamY);
         return FVM2D_OK;
@@ -301,12 +348,10 @@ struct AdjEngine {
             if (it != stored.end()) {
                 s = it->second;  // placed by the forward call's spine
             } else if ((s = take_slot()) >= 0) {
-                std::vector<double> t;
-                int rc = advance(sa, m, slot_t[sa], &t);
+                int res;
+                int rc = advance(sa, m, &res);
+                if (!rc) rc = store_slot(res, s);
                 if (rc) return rc;
-                rc = copy_state(-1, s);
-                if (rc) return rc;
-                slot_t[s] = t;
                 slot_step[s] = a + m;
                 stored[a + m] = s;
             }
@@ -320,9 +365,9 @@ struct AdjEngine {
         }
         // no checkpoint to spare: recompute every state from U_a
         for (int k = len - 1; k >= 1; k--) {
-            int rc = advance(sa, k, slot_t[sa], nullptr);
-            if (rc) return rc;
-            rc = reverse_step(-1);
+            int res;
+            int rc = advance(sa, k, &res);
+            if (!rc) rc = reverse_step(res);
             if (rc) return rc;
         }
         return reverse_step(sa);
@@ -341,12 +386,12 @@ struct AdjEngine {
         if (!launch_chosen) {
             // first forward call: time the candidate launch geometries of the
             // step kernel on this initial state (the autotuner restores it)
-            int rc = copy_state(0, -1);
+            int rc = copy_state(0, kBuf0);
             if (!rc) rc = fwd.autotune(2, nullptr);
             if (rc) return rc;
             launch_chosen = true;
         }
-        slot_t.assign(slots, std::vector<double>(n, 0.0));
+        HIPCHK(hipMemsetAsync(d_slot_t, 0, (size_t)slots * n * sizeof(double), fwd.stream));
         slot_step.assign(slots, -1);
         stored.clear();
         slot_step[0] = 0;
@@ -355,15 +400,13 @@ struct AdjEngine {
         HIPCHK(hipEventRecord(r0, fwd.stream));
         // walk the first spine of the schedule, storing its checkpoints
         int a = 0, len = nsteps, c = slots - 1, sa = 0;
-        std::vector<double> t;
         while (len > 1 && c > 0) {
             const int m = split[len][c];
             const int s = take_slot();
-            int rc = advance(sa, m, slot_t[sa], &t);
+            int res;
+            int rc = advance(sa, m, &res);
+            if (!rc) rc = store_slot(res, s);
             if (rc) return rc;
-            rc = copy_state(-1, s);
-            if (rc) return rc;
-            slot_t[s] = t;
             slot_step[s] = a + m;
             stored[a + m] = s;
             a += m;
@@ -371,18 +414,23 @@ struct AdjEngine {
             c -= 1;
             sa = s;
         }
-        int rc = advance(sa, len, slot_t[sa], &t);
+        int res;
+        int rc = advance(sa, len, &res);
         if (rc) return rc;
         HIPCHK(hipEventRecord(r1, fwd.stream));
-        HIPCHK(hipEventSynchronize(r1));
+        rc = fwd.read_ctl();
+        if (rc) return rc;
+        for (int i = 0; i < n; i++)
+            if (fwd.h_ctl[i].status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite state in the forward run");
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, r0, r1));
         fwd_ms = ms;
         for (int i = 0; i < n; i++) {
             const InstDesc &d = fwd.hd[i];
-            HIPCHK(hipMemcpyAsync(UN[i], d.buf0, 4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice,
+            const double *uN = state_ptr(i, res);
+            HIPCHK(hipMemcpyAsync(UN[i], uN, 4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice,
                                   fwd.stream));
-            adj_launch_prim_to_cons(d.buf0, UN_cons[i], nx[i], ny[i], d.pitch, d.plane, 1.0 / (gamma[i] - 1.0),
+            adj_launch_prim_to_cons(uN, UN_cons[i], nx[i], ny[i], d.pitch, d.plane, 1.0 / (gamma[i] - 1.0),
                                     fwd.stream);
         }
         HIPCHK(hipGetLastError());
@@ -406,12 +454,16 @@ struct AdjEngine {
         recompute_steps = 0;
         reverse_steps = 0;
         rev_ms_sum = 0.0;
+        ring_pending = 0;
+        int rc = build_desc_table();
+        if (rc) return rc;
         HIPCHK(hipMemsetAsync(d_actl, 0, n * sizeof(AdjCtl), fwd.stream));
         hipEvent_t b0, b1;
         HIPCHK(hipEventCreate(&b0));
         HIPCHK(hipEventCreate(&b1));
         HIPCHK(hipEventRecord(b0, fwd.stream));
-        int rc = reverse(0, nsteps, slots - 1, 0);
+        rc = reverse(0, nsteps, slots - 1, 0);
+        if (!rc) rc = drain_ring(0);
         if (rc) {
             hipEventDestroy(b0);
             hipEventDestroy(b1);

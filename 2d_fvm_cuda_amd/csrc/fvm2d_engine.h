@@ -326,14 +326,22 @@ struct Engine {
     // and compute s_max of the initial state.
     // t0: per-instance start times (host array of n values), or null for 0.
     int begin(const double *t0 = nullptr) {
-        HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
-        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
-        for (int i = 0; i < n; i++) launch_speed(exact, d_desc, d_ctl, i, 0, hd[i].ny, stream);
         if (t0) {
+            // h_t0 is rewritten only after the stream has consumed it
+            HIPCHK(hipStreamSynchronize(stream));
             for (int i = 0; i < n; i++) h_t0[i] = t0[i];
             HIPCHK(hipMemcpyAsync(d_t0, h_t0, n * sizeof(double), hipMemcpyHostToDevice, stream));
         }
-        launch_ctl_start(d_desc, d_ctl, n, t0 ? d_t0 : nullptr, stream);
+        return begin_device(t0 ? d_t0 : nullptr);
+    }
+
+    // begin() with the start times in device memory (n values, or null for
+    // 0): enqueues the reset without waiting for the host.
+    int begin_device(const double *d_times) {
+        HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
+        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
+        for (int i = 0; i < n; i++) launch_speed(exact, d_desc, d_ctl, i, 0, hd[i].ny, stream);
+        launch_ctl_start(d_desc, d_ctl, n, d_times, stream);
         HIPCHK(hipGetLastError());
         begun = true;
         return FVM2D_OK;
