@@ -359,9 +359,10 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double iSl = rsq(xl), iSr = rsq(xr);
     const double Sl = xl * iSl, Sr = xr * iSr;
     const double irl = gpl * iSl * iSl, irr = gpr * iSr * iSr;
-    // c_x = 0.5 sqrt(gamma/(p rho)) (p rho_x + rho p_x) = 0.5 gamma/S (...)
-    const double clx = (0.5 * gamma) * iSl * fma(pl, sl.r, rl * sl.p);
-    const double crx = (0.5 * gamma) * iSr * fma(pr, sr.r, rr * sr.p);
+    // c_x = 0.5 sqrt(gamma/(p rho)) (p rho_x + rho p_x) = (gamma/2) (...)/S;
+    // clx and crx below leave out the factor gamma/2, which their uses take
+    const double clx = iSl * fma(pl, sl.r, rl * sl.p);
+    const double crx = iSr * fma(pr, sr.r, rr * sr.p);
     const double du = ul - ur;
     // Relaxation speeds (src/basicFuns.cuh:94-104).  The side F with the
     // lower pressure is relaxed first from the other side's S, then the
@@ -379,8 +380,8 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double icO = csum * t, ics = cO * t;
     const double cl = fL ? cF : cO, cr = fL ? cO : cF;
     const double icl = fL ? icF : icO, icr = fL ? icO : icF;
-    const bool supL = ul - cl * irl >= 0.0;
-    const bool supR = !supL && ur + cr * irr <= 0.0;
+    const bool supL = fma(-cl, irl, ul) >= 0.0;  // u_l - c_l / rho_l >= 0
+    const bool supR = !supL && fma(cr, irr, ur) <= 0.0;
     if (supL || supR) {
         // upwind state with the primitive Euler time derivatives; the two
         // supersonic branches differ only in which side K is upwind.  Kept
@@ -401,7 +402,8 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
         return supL ? 0 : 1;
     }
     const double um = (fma(ul, cl, fma(ur, cr, pl - pr))) * ics;
-    const double pm = fma(du * cl, cr, fma(cl, pr, cr * pl)) * ics;
+    const double clcr = cl * cr;
+    const double pm = fma(clcr, du, fma(cl, pr, cr * pl)) * ics;
     // The two star sub-cases (u* <= 0: right state K = r, s = -1; u* > 0:
     // left state K = l, s = +1) are mirror images of each other except for
     // the 1.5 (u* - u_K) c_Kx term, which keeps its sign in both.
@@ -421,10 +423,11 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     // star sub-cases (src/basicFuns.cuh:151-185) reduce to
     //   Dpi/Dt = c_l c_r (P - R) / (c_l + c_r),
     //   p_t = Dpi/Dt + u* rho* M,   u_t = M + u* rho* Dpi/Dt / c_K^2.
-    const double kl = fma(-0.5 * ul, clx, fma(-cl, sl.un, -sl.p));
-    const double kr = fma(0.5 * ur, crx, fma(cr, sr.un, -sr.p));
+    const double g4 = 0.25 * gamma;  // (1/2) (gamma/2)
+    const double kl = fma(-g4 * ul, clx, fma(-cl, sl.un, -sl.p));
+    const double kr = fma(g4 * ur, crx, fma(cr, sr.un, -sr.p));
     const double P = kl * irl, R = kr * irr;
-    const double Dp = (cl * cr) * ics * (P - R);
+    const double Dp = clcr * ics * (P - R);
     const double M = fma(cl, P, cr * R) * ics;
     const double cirK = cK * irK;
     const double den = fma(s, um - uK, cirK);  // c_K / rho*
@@ -434,7 +437,8 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     // e* = e_K + (p*^2 - p_K^2) / (2 c_K^2)
     const double D2 = (pm - pK) * (pm + pK) * icK2;
     const double e = fma(0.5, D2, eK);
-    const double t1 = fma(cirK * irK, cK * sK.r, fma(s * cK, sK.un, 1.5 * (um - uK) * cKx));
+    // c_K^2/rho_K^2 rho_Kx + s c_K u_Kx + 1.5 (u* - u_K) c_Kx
+    const double t1 = fma(cirK * cirK, sK.r, fma(s * cK, sK.un, (0.75 * gamma) * (um - uK) * cKx));
     // rho*^3 / (rho_K c_K^2) = (c_K / rho_K) / den^3 and rho*^3 / c_K^3 =
     // 1 / den^3; the reference's (c_K / rho* - s u*), with c_K / rho* = den
     // = s (u* - u_K) + c_K / rho_K, reduces to c_K / rho_K - s u_K
@@ -446,7 +450,8 @@ FVM_HD int grp<false>(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const 
     const double p_t = fma(urho, M, Dp);
     const double u_t = fma(urho * icK2, Dp, M);
     // 2 (e_K - e*) / c_K = -D2 / c_K
-    const double e_t = fma(-qK, fma(-D2 * icK, cKx, fma(-pK * icK2, sK.p, eKx)), pm * icK2 * p_t);
+    const double e_t = fma(-qK, fma(-(0.5 * gamma) * D2 * icK, cKx, fma(-pK * icK2, sK.p, eKx)),
+                           pm * icK2 * p_t);
     W = IfState{rho, pm, um, right ? fr.ut : fl.ut, e};
     Wt = IfState{rho_t, p_t, u_t, -qK * sK.ut, e_t};
     return right ? 2 : 3;
@@ -501,14 +506,15 @@ FVM_HD Q update(const Q &q, const Q &FL, const Q &FR, double lambda, const Gas &
         c3 = c3 - (FR.ut - FL.ut) * lambda;
         return Q{c0, (gamma - 1.0) * (c2 - c1 * c1 / c0 / 2.0 - c3 * c3 / c0 / 2.0), c1 / c0, c3 / c0};
     }
+    const double mu = q.r * q.un, mv = q.r * q.ut;  // momenta
     const double c0 = fma(-lambda, FR.r - FL.r, q.r);
-    const double c1 = fma(-lambda, FR.p - FL.p, q.r * q.un);
-    const double E = fma(q.p, G.g1i, 0.5 * q.r * fma(q.un, q.un, q.ut * q.ut));
+    const double c1 = fma(-lambda, FR.p - FL.p, mu);
+    const double E = fma(0.5, fma(mu, q.un, mv * q.ut), q.p * G.g1i);  // p/(g-1) + rho |u|^2/2
     const double c2 = fma(-lambda, FR.un - FL.un, E);
-    const double c3 = fma(-lambda, FR.ut - FL.ut, q.r * q.ut);
+    const double c3 = fma(-lambda, FR.ut - FL.ut, mv);
     const double ir = rcp(c0);
     const double un = c1 * ir, ut = c3 * ir;
-    const double p = G.g1 * fma(-0.5 * ir, fma(c1, c1, c3 * c3), c2);
+    const double p = G.g1 * fma(-0.5, fma(c1, un, c3 * ut), c2);  // (g-1) (E - rho |u|^2/2)
     return Q{c0, p, un, ut};
 }
 
