@@ -40,8 +40,10 @@ __device__ __forceinline__ Q xsweep(const Q &q, const Line &L, const Gas &G, dou
     const Q qL = shr(q), qR = shl(q);
     const Q s = sweep_slope<Exact>(qL, q, qR, L);
     const Q fm = sweep_face_minus<Exact>(q, s, L);
-    const Q fp = sweep_face_plus<Exact>(q, s, L);
-    const Q fpL = shr(fp), sL = shr(s);
+    const Q sL = shr(s);
+    // east face of the west neighbour, from its state (already shifted in)
+    // and its shifted slope: the same operations as on that lane
+    const Q fpL = sweep_face_plus<Exact>(qL, sL, L);
     IfState W, Wt;
     grp<Exact>(fpL, fm, sL, s, G, W, Wt);  // interface on the west face
     const Q F = flux<Exact>(W, Wt, sweep_k<Exact>(dt, L));
