@@ -137,7 +137,6 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
     const double iSl = rsq(Xl), iSr = rsq(Xr);
     const double Sl = Xl * iSl, Sr = Xr * iSr;
     const double irl = gpl * iSl * iSl, irr = gpr * iSr * iSr;
-    const double el = pl * irl * g1i, er = pr * irr * g1i;
     const double zl = plx - pl * irl * rlx, zr = prx - pr * irr * rrx;
     const double elx = zl * irl * g1i, erx = zr * irr * g1i;
     const double sxl = pl * rlx + rl * plx, sxr = pr * rrx + rr * prx;
@@ -228,294 +227,164 @@ FVM_HD void grp_adj(const Q &fl, const Q &fr, const Q &sl, const Q &sr, const Ga
         b_irr -= cr * cr * urx * Wtb.p;
         b_urx -= cr * cr * irr * Wtb.p;
     } else {  // star region
-        // ---- forward of the common star part
+        // ---- forward: both sub-cases through side K (u* <= 0: K = r, s = -1;
+        // else K = l, s = +1) in the closed form of grp<false>: with
+        // P = k_l/rho_l, R = k_r/rho_r and M = (c_l P + c_r R)/(c_l + c_r),
+        // Dpi/Dt = c_l c_r (P - R)/(c_l + c_r), p_t = Dpi/Dt + u* rho* M,
+        // u_t = M + u* rho* Dpi/Dt / c_K^2, rho_t = (-u* (c_K/rho_K) t1 +
+        // (c_K/rho_K - s u_K) Dpi/Dt) / den^3 with den = c_K / rho*
         const double ics = rcp(cl + cr);
         const double Nu = ul * cl + ur * cr + pl - pr;
         const double um = Nu * ics;
         const double Np = du * cl * cr + cl * pr + cr * pl;
         const double pm = Np * ics;
-        const double Ql = -2.0 * cl * ulx - ul * clx - 2.0 * plx;
-        const double Qr = 2.0 * cr * urx + ur * crx - 2.0 * prx;
         const double kl = -plx - cl * ulx - 0.5 * ul * clx;
         const double kr = -prx + cr * urx + 0.5 * ur * crx;
-        const double dl = cl * irl * Ql, dr = cr * irr * Qr;
-        const double ND = cr * dl - cl * dr;
-        const double Dp = ND * 0.5 * ics;
-        double b_um = Wb.un, b_pm = Wb.p, b_ics = 0, b_Dp = 0, b_kl = 0, b_kr = 0;
-        const double b_rho_t = Wtb.r, b_v_t = Wtb.ut, b_e_t = Wtb.e, b_u_t = Wtb.un;
+        const double Pl = kl * irl, Rr = kr * irr;
+        const double D = Pl - Rr;
+        const double clcr = cl * cr;
+        const double Dp = clcr * ics * D;
+        const double Mn = cl * Pl + cr * Rr;
+        const double M = Mn * ics;
+        const bool right = um <= 0.0;
+        const double s = right ? -1.0 : 1.0;
+        const double cK = right ? cr : cl, icK = right ? icr : icl, irK = right ? irr : irl;
+        const double uK = right ? ur : ul, pK = right ? pr : pl;
+        const double eKx = right ? erx : elx, cKx = right ? crx : clx;
+        const double rxK = right ? rrx : rlx, uxK = right ? urx : ulx;
+        const double pxK = right ? prx : plx, vxK = right ? vrx : vlx;
+        const double cirK = cK * irK;
+        const double den = s * (um - uK) + cirK;
+        const double iden = rcp(den);
+        const double rho = cK * iden;
+        const double icK2 = icK * icK;
+        const double dpm = pm - pK, spm = pm + pK;
+        const double D2 = dpm * spm * icK2;  // (p*^2 - p_K^2) / c_K^2 = 2 (e* - e_K)
+        const double t1 = cirK * cirK * rxK + s * cK * uxK + 1.5 * (um - uK) * cKx;
+        const double iden3 = iden * iden * iden;
+        const double umc = um * cirK;
+        const double G1 = cirK - s * uK;
+        const double Hn = -umc * t1 + G1 * Dp;  // rho_t = iden3 Hn
+        const double urho = um * rho;
+        const double qK = umc * iden;  // u* rho* / rho_K
+        const double p_t = Dp + urho * M;
+        const double Ye = -D2 * icK * cKx - pK * icK2 * pxK + eKx;  // e_t = -qK Ye + pm icK2 p_t
+        // ---- reverse
+        double b_rho = Wb.r, b_pm = Wb.p, b_um = Wb.un;
+        const double b_e = Wb.e, b_vK = Wb.ut;
+        const double b_rho_t = Wtb.r, b_u_t = Wtb.un, b_v_t = Wtb.ut, b_e_t = Wtb.e;
         double b_p_t = Wtb.p;
-        double b_rho = Wb.r, b_e = Wb.e;
-        if (um <= 0.0) {
-            // ---- forward of the right sub-case
-            const double den = ur - um + cr * irr;
-            const double iden = rcp(den);
-            const double rho = cr * iden;
-            const double irho = den * icr;
-            const double icr2 = icr * icr;
-            const double dpm = pm - pr, spm = pm + pr;
-            const double e = er + dpm * spm * 0.5 * icr2;
-            const double r3 = rho * rho * rho;
-            const double t1 = cr * cr * irr * irr * rrx - cr * urx + 1.5 * (um - ur) * crx;
-            const double X1 = -um * r3 * irr * icr2;
-            const double Z = um + cr * irho;
-            const double X2 = r3 * icr2 * icr * Z;
-            const double urho = um * rho;
-            const double fA = 1.0 + urho * icr;
-            const double A = cl * irl * fA * kl;
-            const double fB = 1.0 - urho * icl;
-            const double B = cr * irr * fB * kr;
-            const double p_t = ics * (cr * A - cl * B);
-            const double fU = 1.0 - urho * cl * icr2;
-            const double Pu = A + cr * irr * fU * kr;
-            const double Pp = cr * A - cl * B;
-            const double Ye = erx - pr * icr2 * prx + 2.0 * (er - e) * icr * crx;
-            // ---- reverse
-            b_vr += Wb.ut;
-            double b_urho = 0, b_icr2 = 0, b_A = 0, b_B = 0, b_r3 = 0, b_irho = 0, b_den = 0;
-            // v_t = -um rho irr vrx
-            b_um -= rho * irr * vrx * b_v_t;
-            b_rho -= um * irr * vrx * b_v_t;
-            b_irr -= um * rho * vrx * b_v_t;
-            b_vrx -= um * rho * irr * b_v_t;
-            // e_t = -urho irr Ye + pm icr2 p_t
-            b_urho -= irr * Ye * b_e_t;
-            b_irr -= urho * Ye * b_e_t;
-            const double b_Ye = -urho * irr * b_e_t;
-            b_pm += icr2 * p_t * b_e_t;
-            b_icr2 += pm * p_t * b_e_t;
-            b_p_t += pm * icr2 * b_e_t;
-            b_erx += b_Ye;
-            b_pr -= icr2 * prx * b_Ye;
-            b_icr2 -= pr * prx * b_Ye;
-            b_prx -= pr * icr2 * b_Ye;
-            b_er += 2.0 * icr * crx * b_Ye;
-            b_e -= 2.0 * icr * crx * b_Ye;
-            b_icr += 2.0 * (er - e) * crx * b_Ye;
-            b_crx += 2.0 * (er - e) * icr * b_Ye;
-            // u_t = ics (A + cr irr fU kr)
-            b_ics += Pu * b_u_t;
-            const double b_Pu = ics * b_u_t;
-            b_A += b_Pu;
-            b_cr += irr * fU * kr * b_Pu;
-            b_irr += cr * fU * kr * b_Pu;
-            const double b_fU = cr * irr * kr * b_Pu;
-            b_kr += cr * irr * fU * b_Pu;
-            b_urho -= cl * icr2 * b_fU;
-            b_cl -= urho * icr2 * b_fU;
-            b_icr2 -= urho * cl * b_fU;
-            // p_t = ics (cr A - cl B)
-            b_ics += Pp * b_p_t;
-            const double b_Pp = ics * b_p_t;
-            b_cr += A * b_Pp;
-            b_A += cr * b_Pp;
-            b_cl -= B * b_Pp;
-            b_B -= cl * b_Pp;
-            // B = cr irr fB kr, fB = 1 - urho icl
-            b_cr += irr * fB * kr * b_B;
-            b_irr += cr * fB * kr * b_B;
-            const double b_fB = cr * irr * kr * b_B;
-            b_kr += cr * irr * fB * b_B;
-            b_urho -= icl * b_fB;
-            b_icl -= urho * b_fB;
-            // A = cl irl fA kl, fA = 1 + urho icr
-            b_cl += irl * fA * kl * b_A;
-            b_irl += cl * fA * kl * b_A;
-            const double b_fA = cl * irl * kl * b_A;
-            b_kl += cl * irl * fA * b_A;
-            b_urho += icr * b_fA;
-            b_icr += urho * b_fA;
-            // urho = um rho
-            b_um += rho * b_urho;
-            b_rho += um * b_urho;
-            // rho_t = X1 t1 + X2 Dp
-            const double b_X2 = Dp * b_rho_t;
-            b_Dp += X2 * b_rho_t;
-            b_r3 += icr2 * icr * Z * b_X2;
-            b_icr2 += r3 * icr * Z * b_X2;
-            b_icr += r3 * icr2 * Z * b_X2;
-            const double b_Z = r3 * icr2 * icr * b_X2;
-            b_um += b_Z;
-            b_cr += irho * b_Z;
-            b_irho += cr * b_Z;
-            const double b_X1 = t1 * b_rho_t;
-            const double b_t1 = X1 * b_rho_t;
-            b_um -= r3 * irr * icr2 * b_X1;
-            b_r3 -= um * irr * icr2 * b_X1;
-            b_irr -= um * r3 * icr2 * b_X1;
-            b_icr2 -= um * r3 * irr * b_X1;
-            // t1 = cr^2 irr^2 rrx - cr urx + 1.5 (um - ur) crx
-            b_cr += (2.0 * cr * irr * irr * rrx - urx) * b_t1;
-            b_irr += 2.0 * cr * cr * irr * rrx * b_t1;
-            b_rrx += cr * cr * irr * irr * b_t1;
-            b_urx -= cr * b_t1;
-            b_um += 1.5 * crx * b_t1;
-            b_ur -= 1.5 * crx * b_t1;
-            b_crx += 1.5 * (um - ur) * b_t1;
-            // r3 = rho^3
-            b_rho += 3.0 * rho * rho * b_r3;
-            // e = er + dpm spm icr2 / 2
-            b_er += b_e;
-            const double b_dpm = spm * 0.5 * icr2 * b_e;
-            const double b_spm = dpm * 0.5 * icr2 * b_e;
-            b_icr2 += dpm * spm * 0.5 * b_e;
-            b_pm += b_dpm + b_spm;
-            b_pr += -b_dpm + b_spm;
-            b_icr += 2.0 * icr * b_icr2;
-            // irho = den icr ; rho = cr / den
-            b_den += icr * b_irho;
-            b_icr += den * b_irho;
-            b_cr += iden * b_rho;
-            b_den -= iden * iden * cr * b_rho;
-            // den = ur - um + cr irr
-            b_ur += b_den;
-            b_um -= b_den;
-            b_cr += irr * b_den;
-            b_irr += cr * b_den;
-        } else {
-            // ---- forward of the left sub-case
-            const double den = um - ul + cl * irl;
-            const double iden = rcp(den);
-            const double rho = cl * iden;
-            const double irho = den * icl;
-            const double icl2 = icl * icl;
-            const double dpm = pm - pl, spm = pm + pl;
-            const double e = el + dpm * spm * 0.5 * icl2;
-            const double r3 = rho * rho * rho;
-            const double t1 = cl * cl * irl * irl * rlx + cl * ulx + 1.5 * (um - ul) * clx;
-            const double X1 = -um * r3 * irl * icl2;
-            const double Z = um - cl * irho;
-            const double X2 = -r3 * icl2 * icl * Z;
-            const double urho = um * rho;
-            const double fA = 1.0 + urho * icr;
-            const double A = cl * irl * fA * kl;
-            const double fB = 1.0 - urho * icl;
-            const double B = cr * irr * fB * kr;
-            const double p_t = ics * (cr * A - cl * B);
-            const double fU = 1.0 + urho * cr * icl2;
-            const double Pu = cl * irl * fU * kl + B;
-            const double Pp = cr * A - cl * B;
-            const double Ye = elx - pl * icl2 * plx + 2.0 * (el - e) * icl * clx;
-            // ---- reverse
-            b_vl += Wb.ut;
-            double b_urho = 0, b_icl2 = 0, b_A = 0, b_B = 0, b_r3 = 0, b_irho = 0, b_den = 0;
-            b_um -= rho * irl * vlx * b_v_t;
-            b_rho -= um * irl * vlx * b_v_t;
-            b_irl -= um * rho * vlx * b_v_t;
-            b_vlx -= um * rho * irl * b_v_t;
-            b_urho -= irl * Ye * b_e_t;
-            b_irl -= urho * Ye * b_e_t;
-            const double b_Ye = -urho * irl * b_e_t;
-            b_pm += icl2 * p_t * b_e_t;
-            b_icl2 += pm * p_t * b_e_t;
-            b_p_t += pm * icl2 * b_e_t;
-            b_elx += b_Ye;
-            b_pl -= icl2 * plx * b_Ye;
-            b_icl2 -= pl * plx * b_Ye;
-            b_plx -= pl * icl2 * b_Ye;
-            b_el += 2.0 * icl * clx * b_Ye;
-            b_e -= 2.0 * icl * clx * b_Ye;
-            b_icl += 2.0 * (el - e) * clx * b_Ye;
-            b_clx += 2.0 * (el - e) * icl * b_Ye;
-            // u_t = ics (cl irl fU kl + B)
-            b_ics += Pu * b_u_t;
-            const double b_Pu = ics * b_u_t;
-            b_cl += irl * fU * kl * b_Pu;
-            b_irl += cl * fU * kl * b_Pu;
-            const double b_fU = cl * irl * kl * b_Pu;
-            b_kl += cl * irl * fU * b_Pu;
-            b_B += b_Pu;
-            b_urho += cr * icl2 * b_fU;
-            b_cr += urho * icl2 * b_fU;
-            b_icl2 += urho * cr * b_fU;
-            // p_t = ics (cr A - cl B)
-            b_ics += Pp * b_p_t;
-            const double b_Pp = ics * b_p_t;
-            b_cr += A * b_Pp;
-            b_A += cr * b_Pp;
-            b_cl -= B * b_Pp;
-            b_B -= cl * b_Pp;
-            b_cr += irr * fB * kr * b_B;
-            b_irr += cr * fB * kr * b_B;
-            const double b_fB = cr * irr * kr * b_B;
-            b_kr += cr * irr * fB * b_B;
-            b_urho -= icl * b_fB;
-            b_icl -= urho * b_fB;
-            b_cl += irl * fA * kl * b_A;
-            b_irl += cl * fA * kl * b_A;
-            const double b_fA = cl * irl * kl * b_A;
-            b_kl += cl * irl * fA * b_A;
-            b_urho += icr * b_fA;
-            b_icr += urho * b_fA;
-            b_um += rho * b_urho;
-            b_rho += um * b_urho;
-            // rho_t = X1 t1 + X2 Dp, X2 = -r3 icl2 icl Z, Z = um - cl irho
-            const double b_X2 = Dp * b_rho_t;
-            b_Dp += X2 * b_rho_t;
-            b_r3 -= icl2 * icl * Z * b_X2;
-            b_icl2 -= r3 * icl * Z * b_X2;
-            b_icl -= r3 * icl2 * Z * b_X2;
-            const double b_Z = -r3 * icl2 * icl * b_X2;
-            b_um += b_Z;
-            b_cl -= irho * b_Z;
-            b_irho -= cl * b_Z;
-            const double b_X1 = t1 * b_rho_t;
-            const double b_t1 = X1 * b_rho_t;
-            b_um -= r3 * irl * icl2 * b_X1;
-            b_r3 -= um * irl * icl2 * b_X1;
-            b_irl -= um * r3 * icl2 * b_X1;
-            b_icl2 -= um * r3 * irl * b_X1;
-            // t1 = cl^2 irl^2 rlx + cl ulx + 1.5 (um - ul) clx
-            b_cl += (2.0 * cl * irl * irl * rlx + ulx) * b_t1;
-            b_irl += 2.0 * cl * cl * irl * rlx * b_t1;
-            b_rlx += cl * cl * irl * irl * b_t1;
-            b_ulx += cl * b_t1;
-            b_um += 1.5 * clx * b_t1;
-            b_ul -= 1.5 * clx * b_t1;
-            b_clx += 1.5 * (um - ul) * b_t1;
-            b_rho += 3.0 * rho * rho * b_r3;
-            b_el += b_e;
-            const double b_dpm = spm * 0.5 * icl2 * b_e;
-            const double b_spm = dpm * 0.5 * icl2 * b_e;
-            b_icl2 += dpm * spm * 0.5 * b_e;
-            b_pm += b_dpm + b_spm;
-            b_pl += -b_dpm + b_spm;
-            b_icl += 2.0 * icl * b_icl2;
-            b_den += icl * b_irho;
-            b_icl += den * b_irho;
-            b_cl += iden * b_rho;
-            b_den -= iden * iden * cl * b_rho;
-            // den = um - ul + cl irl
-            b_um += b_den;
-            b_ul -= b_den;
-            b_cl += irl * b_den;
-            b_irl += cl * b_den;
-        }
-        // ---- reverse of the common star part
-        // Dp = ND ics / 2, ND = cr dl - cl dr
-        const double b_ND = 0.5 * ics * b_Dp;
-        b_ics += ND * 0.5 * b_Dp;
-        b_cr += dl * b_ND;
-        const double b_dl = cr * b_ND;
-        b_cl -= dr * b_ND;
-        const double b_dr = -cl * b_ND;
-        // dr = cr irr Qr, Qr = 2 cr urx + ur crx - 2 prx
-        b_cr += irr * Qr * b_dr;
-        b_irr += cr * Qr * b_dr;
-        const double b_Qr = cr * irr * b_dr;
-        b_cr += 2.0 * urx * b_Qr;
-        b_urx += 2.0 * cr * b_Qr;
-        b_ur += crx * b_Qr;
-        b_crx += ur * b_Qr;
-        b_prx -= 2.0 * b_Qr;
-        // dl = cl irl Ql, Ql = -2 cl ulx - ul clx - 2 plx
-        b_cl += irl * Ql * b_dl;
-        b_irl += cl * Ql * b_dl;
-        const double b_Ql = cl * irl * b_dl;
-        b_cl -= 2.0 * ulx * b_Ql;
-        b_ulx -= 2.0 * cl * b_Ql;
-        b_ul -= clx * b_Ql;
-        b_clx -= ul * b_Ql;
-        b_plx -= 2.0 * b_Ql;
+        // v_t = -qK vxK
+        double b_qK = -vxK * b_v_t;
+        const double b_vxK = -qK * b_v_t;
+        // e_t = -qK Ye + pm icK2 p_t
+        b_qK -= Ye * b_e_t;
+        const double b_Ye = -qK * b_e_t;
+        b_pm += icK2 * p_t * b_e_t;
+        double b_icK2 = pm * p_t * b_e_t;
+        b_p_t += pm * icK2 * b_e_t;
+        // Ye = -D2 icK cKx - pK icK2 pxK + eKx
+        double b_D2 = -icK * cKx * b_Ye;
+        double b_icK = -D2 * cKx * b_Ye;
+        double b_cKx = -D2 * icK * b_Ye;
+        double b_pK = -icK2 * pxK * b_Ye;
+        b_icK2 -= pK * pxK * b_Ye;
+        const double b_pxK = -pK * icK2 * b_Ye;
+        const double b_eKx = b_Ye;
+        // u_t = M + urho icK2 Dp
+        double b_M = b_u_t;
+        double b_urho = icK2 * Dp * b_u_t;
+        b_icK2 += urho * Dp * b_u_t;
+        double b_Dp = urho * icK2 * b_u_t;
+        // p_t = Dp + urho M
+        b_Dp += b_p_t;
+        b_urho += M * b_p_t;
+        b_M += urho * b_p_t;
+        // qK = umc iden
+        double b_umc = iden * b_qK;
+        double b_iden = umc * b_qK;
+        // urho = um rho
+        b_um += rho * b_urho;
+        b_rho += um * b_urho;
+        // rho_t = iden3 Hn, Hn = -umc t1 + G1 Dp
+        const double b_iden3 = Hn * b_rho_t;
+        const double b_Hn = iden3 * b_rho_t;
+        b_umc -= t1 * b_Hn;
+        const double b_t1 = -umc * b_Hn;
+        b_Dp += G1 * b_Hn;
+        // G1 = cirK - s uK
+        double b_cirK = Dp * b_Hn;
+        double b_uK = -s * Dp * b_Hn;
+        // umc = um cirK
+        b_um += cirK * b_umc;
+        b_cirK += um * b_umc;
+        // iden3 = iden^3
+        b_iden += 3.0 * iden * iden * b_iden3;
+        // t1 = cirK^2 rxK + s cK uxK + 1.5 (um - uK) cKx
+        b_cirK += 2.0 * cirK * rxK * b_t1;
+        const double b_rxK = cirK * cirK * b_t1;
+        double b_cK = s * uxK * b_t1;
+        const double b_uxK = s * cK * b_t1;
+        b_um += 1.5 * cKx * b_t1;
+        b_uK -= 1.5 * cKx * b_t1;
+        b_cKx += 1.5 * (um - uK) * b_t1;
+        // e = eK + D2 / 2
+        b_D2 += 0.5 * b_e;
+        // D2 = dpm spm icK2
+        b_pm += (spm + dpm) * icK2 * b_D2;
+        b_pK += (dpm - spm) * icK2 * b_D2;
+        b_icK2 += dpm * spm * b_D2;
+        // icK2 = icK^2
+        b_icK += 2.0 * icK * b_icK2;
+        // rho = cK iden
+        b_cK += iden * b_rho;
+        b_iden += cK * b_rho;
+        // iden = 1 / den; den = s (um - uK) + cirK
+        const double b_den = -iden * iden * b_iden;
+        b_um += s * b_den;
+        b_uK -= s * b_den;
+        b_cirK += b_den;
+        // cirK = cK irK
+        b_cK += irK * b_cirK;
+        const double b_irK = cK * b_cirK;
+        // side K back to the left or right quantities
+        auto side = [right](double b, double &to_l, double &to_r) {
+            to_l += right ? 0.0 : b;
+            to_r += right ? b : 0.0;
+        };
+        side(b_cK, b_cl, b_cr);
+        side(b_icK, b_icl, b_icr);
+        side(b_irK, b_irl, b_irr);
+        side(b_uK, b_ul, b_ur);
+        side(b_pK, b_pl, b_pr);
+        side(b_e, b_el, b_er);
+        side(b_eKx, b_elx, b_erx);
+        side(b_cKx, b_clx, b_crx);
+        side(b_rxK, b_rlx, b_rrx);
+        side(b_uxK, b_ulx, b_urx);
+        side(b_pxK, b_plx, b_prx);
+        side(b_vxK, b_vlx, b_vrx);
+        side(b_vK, b_vl, b_vr);
+        // M = Mn ics, Mn = cl Pl + cr Rr
+        const double b_Mn = ics * b_M;
+        double b_ics = Mn * b_M;
+        b_cl += Pl * b_Mn;
+        double b_Pl = cl * b_Mn;
+        b_cr += Rr * b_Mn;
+        double b_Rr = cr * b_Mn;
+        // Dp = clcr ics D, D = Pl - Rr
+        const double b_clcr = ics * D * b_Dp;
+        b_ics += clcr * D * b_Dp;
+        const double b_D = clcr * ics * b_Dp;
+        b_cl += cr * b_clcr;
+        b_cr += cl * b_clcr;
+        b_Pl += b_D;
+        b_Rr -= b_D;
+        // Pl = kl irl, Rr = kr irr
+        const double b_kl = irl * b_Pl;
+        b_irl += kl * b_Pl;
+        const double b_kr = irr * b_Rr;
+        b_irr += kr * b_Rr;
         // kr = -prx + cr urx + ur crx / 2
         b_prx -= b_kr;
         b_cr += urx * b_kr;
