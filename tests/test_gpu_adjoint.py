@@ -124,6 +124,35 @@ def _workload_gradient(adjmod, problems, n, steps, vel_modes):
     return adj, C0, target, x.grad.numpy()
 
 
+def test_repeated_calls_reproduce_the_gradient(gpu, problems, adjmod):
+    """An odd step count leaves the adjoint buffers swapped after a backward
+    call, and recomputed states end in either of the engine's two time-loop
+    buffers: a second backward, and a forward/backward pair on another state
+    and back, must give the same gradients bit for bit."""
+    C0 = smooth_state(problems, 52, 38, seed=4)
+    C1 = smooth_state(problems, 52, 38, seed=6)
+    gN = np.random.default_rng(1).normal(size=C0.shape)
+    adj = adjmod.Adjoint([(38, 52)], [{}], nsteps=13, checkpoint_slots=3)
+    U0 = adj.forward([C0])[0]
+    g0 = adj.backward([gN])[0]
+    g0b = adj.backward([gN])[0]
+    adj.forward([C1])
+    g1 = adj.backward([gN])[0]
+    U0c = adj.forward([C0])[0]
+    g0c = adj.backward([gN])[0]
+    adj.close()
+    fresh = adjmod.Adjoint([(38, 52)], [{}], nsteps=13, checkpoint_slots=3)
+    fresh.forward([C1])
+    g1f = fresh.backward([gN])[0]
+    fresh.close()
+    assert np.isfinite(g0).all() and np.abs(g0).max() > 0
+    assert np.array_equal(g0b, g0)
+    assert np.array_equal(U0c, U0)
+    assert np.array_equal(g0c, g0)
+    assert np.array_equal(g1, g1f)
+    assert not np.array_equal(g1, g0)
+
+
 def test_adjoint_workload_fd_differentiable_points(gpu, problems, adjmod):
     """The adjoint workload on 512^2 for 400 steps with binomial
     checkpointing and the loss 1/2 sum (rho_N - rho_target)^2, with seeded
