@@ -219,6 +219,22 @@ struct Engine {
             for (int k = 0; k < cnt; k++) tasks.push_back(make_int2(i, k));
         }
         ntasks = (int)tasks.size();
+        // Task order of one mesh whose tasks fill one round of two waves per
+        // SIMD: the dispatcher gives the first 4 * cu_count positions one
+        // SIMD each and puts position S + k on the SIMD of about position k
+        // (measured).  The older wave of a SIMD takes most of its issue slots,
+        // so a SIMD finishes after about the sum of its two tasks, and the
+        // slowest SIMD sets the launch time.  In row-major order the second
+        // wave is the task 30 strips further south in about the same
+        // columns, and costly regions of a mesh (supersonic interfaces run an
+        // extra branch; edge windows write ghost cells) are wide and pair up.
+        // Reversing the second round pairs each task with its mirror image
+        // through the mesh centre instead (2.5% less time per 2000^2 step).
+        // The order decides only which wave computes which task.
+        if (n == 1 && !persistent) {
+            const int S = 4 * cu_count;
+            if (ntasks > S && ntasks <= 2 * S) std::reverse(tasks.begin() + S, tasks.end());
+        }
         if (d_tasks) HIPCHK(hipFree(d_tasks));
         d_tasks = nullptr;
         HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int2)));
