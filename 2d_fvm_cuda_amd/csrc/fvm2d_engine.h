@@ -498,6 +498,23 @@ struct Engine {
                 const int r = (int)((rows_max + strips - 1) / strips);
                 if (r >= 4 && r <= 96) Rs.push_back(r);
             }
+            // several meshes: for k rounds, the lowest height whose task
+            // count still fits k rounds (the last round full, not a few
+            // waves running alone)
+            auto tasks_at = [&](int r) {
+                long long t = 0;
+                for (int i = 0; i < n; i++) t += (long long)hd[i].nw * ((hd[i].ny - 4 + r - 1) / r);
+                return t;
+            };
+            for (int k = 1; k <= 12 && n > 1; k++) {
+                for (int r = 4; r <= 96; r++) {
+                    const long long t = tasks_at(r);
+                    if (t <= k * slots) {
+                        if (t * 10 > k * slots * 8) Rs.push_back(r);
+                        break;
+                    }
+                }
+            }
             std::sort(Rs.begin(), Rs.end());
             Rs.erase(std::unique(Rs.begin(), Rs.end()), Rs.end());
         }
