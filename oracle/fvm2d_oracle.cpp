@@ -3,6 +3,14 @@
 // Each routine cites the reference lines it follows; expression trees keep
 // the reference's left-to-right evaluation order so that, compiled with
 // -ffp-contract=off, every result is bit-reproducible.
+//
+// Pinned by tests/test_oracle_pins.py: flux, limiter, cons/prim, wave speed,
+// the GRP's supersonic branches and star values (Suliciu relaxation
+// solution), its continuous-data limit, mirror symmetry, the ghost presets,
+// the rotation and the whole-solve invariants.  Parity unpinned: the
+// star-region time derivatives for discontinuous data
+// (src/basicFuns.cuh:151-185) beyond that limit and those identities; the
+// reference cannot be built here and ships no tests or vectors for them.
 #include "fvm2d_oracle.h"
 
 #include <algorithm>
