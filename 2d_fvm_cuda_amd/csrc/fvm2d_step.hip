@@ -92,11 +92,16 @@ __device__ __forceinline__ void store4(double *base, long long plane, long long 
 // ghost rows above and below column c (j in the first or last two
 // interior rows) and the ghost columns beside row j (c on the first or
 // last interior column).  Every ghost cell of the preset has its source
-// or owner in exactly one of these sets.
+// or owner in exactly one of these sets.  Returns the largest signal-speed
+// key among the written ghost cells; the caller folds it into its own
+// maximum.  (Passed by reference, that maximum would live in private
+// memory across the call, and its private-memory round trip made every
+// row of the march wait for all of the previous row's stores.)
 template <bool Exact>
-__device__ __noinline__ void emit_ghosts(const InstDesc &d, double *dst, int j, int c, double r, double p,
-                                         double u, double v, double t_new, unsigned long long &mx,
-                                         int corner) {
+__device__ __noinline__ unsigned long long emit_ghosts(const InstDesc &d, double *dst, int j, int c,
+                                                       double r, double p, double u, double v,
+                                                       double t_new, int corner) {
+    unsigned long long mx = 0ull;
     const int nx = d.nx, ny = d.ny;
     const int rows[5] = {j, 0, 1, ny - 2, ny - 1};
     const int cols[5] = {c, 0, 1, nx - 2, nx - 1};
@@ -125,6 +130,7 @@ __device__ __noinline__ void emit_ghosts(const InstDesc &d, double *dst, int j, 
             }
         }
     }
+    return mx;
 }
 
 // Outflow preset (src/ghostCellPresets.cuh:106-145): every ghost cell is a
@@ -249,7 +255,11 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
                 if (d.strategy == kOutflow)
                     emit_outflow(d, dst, j, col, r, p, u, v);
                 else
-                    emit_ghosts<Exact>(d, dst, j, col, r, p, u, v, t_new, mx, edge_row && edge_col);
+                {
+                    const unsigned long long gm =
+                        emit_ghosts<Exact>(d, dst, j, col, r, p, u, v, t_new, edge_row && edge_col);
+                    mx = gm > mx ? gm : mx;
+                }
             }
         }
     }
