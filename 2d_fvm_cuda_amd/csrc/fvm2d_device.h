@@ -39,8 +39,8 @@ struct InstDesc {
     long long plane;   // doubles per plane = pitch * ny
     int nx, ny, pitch;
     int nw;            // column windows: ceil((nx - 4) / kOwn)
-    int ns;            // row strips:    ceil((ny - 4) / R)
-    int R;             // strip height (output rows per wave)
+    int ntask;         // tasks of this instance in the launch layout
+    int R;             // target strip height (output rows per wave)
     int strategy;
     int max_steps;     // < 0: no step limit
     int task_base;     // first task index of this instance

@@ -72,7 +72,7 @@ struct Engine {
     Sched *d_sched = nullptr;
     CtlSlot *d_slots = nullptr;      // n * kCtlSlots end-of-task arrival slots
     double *d_t0 = nullptr, *h_t0 = nullptr;  // per-instance start times
-    int2 *d_tasks = nullptr;
+    int4 *d_tasks = nullptr;
     int ntasks = 0;
     int R = 16, wpb = 4, persistent = 0, grid = 0;
     int cu_count = 256;
@@ -209,14 +209,47 @@ struct Engine {
         if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
         if (persist >= 0) persistent = persist ? 1 : 0;
         if (R < 1) return fail(FVM2D_ERR_ARG, "strip rows must be >= 1");
-        std::vector<int2> tasks;
+        std::vector<int4> tasks;
+        // resident waves of the launch: blocks per CU x waves per block x CUs
+        const long long slots =
+            (long long)std::max(1, step_kernel_occupancy(exact, wpb)) * wpb * cu_count;
         for (int i = 0; i < n; i++) {
             InstDesc &d = hd[i];
             d.R = R;
-            d.ns = (d.ny - 4 + R - 1) / R;
             d.task_base = (int)tasks.size();
-            const int cnt = d.nw * d.ns;
-            for (int k = 0; k < cnt; k++) tasks.push_back(make_int2(i, k));
+            const int H = d.ny - 4;
+            const int ns = (H + R - 1) / R;
+            // One mesh whose strips of R rows leave the last round of
+            // resident waves partly empty (at 2000 x 2000 and R = 34: 2006
+            // tasks for 2048 wave slots, so ten CUs run half their waves):
+            // cut the windows into more, slightly lower strips so that the
+            // task count fills the round exactly.  Windows 0, nw-1, 1,
+            // nw-2, ... take the extra strips (edge windows also write the
+            // ghost cells).  The cut decides only which wave computes
+            // which rows.
+            const long long T0 = (long long)d.nw * ns;
+            const long long k = (T0 + slots - 1) / slots;
+            const long long T = k * slots;
+            if (n == 1 && !persistent && k <= 2 && T0 < T && T0 * 10 >= T * 8 &&
+                T <= (long long)d.nw * (H / 4)) {
+                const int base = (int)(T / d.nw), extra = (int)(T % d.nw);
+                std::vector<int> nsw(d.nw, base);
+                for (int e = 0; e < extra; e++) nsw[(e & 1) ? d.nw - 1 - e / 2 : e / 2]++;
+                for (int s = 0; s <= base; s++)
+                    for (int w = 0; w < d.nw; w++) {
+                        const int m = nsw[w];
+                        if (s >= m) continue;
+                        tasks.push_back(make_int4(i, w, 2 + (int)((long long)s * H / m),
+                                                  2 + (int)((long long)(s + 1) * H / m)));
+                    }
+            } else {
+                for (int s = 0; s < ns; s++)
+                    for (int w = 0; w < d.nw; w++) {
+                        const int r0 = 2 + s * R;
+                        tasks.push_back(make_int4(i, w, r0, std::min(d.ny - 2, r0 + R)));
+                    }
+            }
+            d.ntask = (int)tasks.size() - d.task_base;
         }
         ntasks = (int)tasks.size();
         // Task order of one mesh whose tasks fill one round of two waves per
@@ -237,8 +270,8 @@ struct Engine {
         }
         if (d_tasks) HIPCHK(hipFree(d_tasks));
         d_tasks = nullptr;
-        HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int2)));
-        HIPCHK(hipMemcpyAsync(d_tasks, tasks.data(), ntasks * sizeof(int2), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int4)));
+        HIPCHK(hipMemcpyAsync(d_tasks, tasks.data(), ntasks * sizeof(int4), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, stream));
         const int blocks_needed = (ntasks + wpb - 1) / wpb;
         if (persistent) {

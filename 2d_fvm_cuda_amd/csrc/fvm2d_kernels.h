@@ -7,7 +7,8 @@
 
 namespace fvm2d {
 
-void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks, int ntasks,
+// A task is one wave's strip: {instance, column window, first row, end row}.
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream);
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
                   long long nrows, hipStream_t stream);

@@ -191,8 +191,11 @@ __device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_
 }
 
 // ---------------------------------------------------------------- the step
+// One task: window w (columns w * kOwn .. + kOwn - 1 owned), output rows
+// [r0, r1); local is the task's index among its instance's tasks.
 template <bool Exact>
-__device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
+__device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, int local,
+                          int lane) {
     const int nx = d.nx, ny = d.ny;
     const long long P = d.plane;
     const double t = ctl->t, dt = ctl->dt;
@@ -204,11 +207,8 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
     const double lamx = dt / LX.h, lamy = dt / LY.h;
     const double t_new = t + dt;
 
-    const int w = local % d.nw, s = local / d.nw;
     const int col = w * kOwn + lane;
     const int colc = col < nx ? col : nx - 1;
-    const int r0 = 2 + s * d.R;
-    const int r1 = min(ny - 2, r0 + d.R);
     const bool own = lane >= 2 && lane < kWave - 2 && col <= nx - 3;
 
     auto load = [&](int j) -> Q {  // X-order
@@ -267,7 +267,7 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
     mx = wave_max_u64(mx);
     if (lane == 0) {
         // two-level arrival: task -> its slot -> the instance
-        const int nt = d.nw * d.ns;
+        const int nt = d.ntask;
         const int slot = local % kCtlSlots;
         const int nslots = nt < kCtlSlots ? nt : kCtlSlots;
         const int per = nt / kCtlSlots + (slot < nt % kCtlSlots ? 1 : 0);
@@ -296,7 +296,7 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int local, int lane) {
 template <bool Exact>
 __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restrict__ descs,
                                                        Ctl *__restrict__ ctls,
-                                                       const int2 *__restrict__ tasks, int ntasks,
+                                                       const int4 *__restrict__ tasks, int ntasks,
                                                        Sched *sched, int persistent) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
@@ -310,11 +310,15 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
         tk = blockIdx.x * wpb + wid;
     }
     while (tk < ntasks) {
-        const int2 task = tasks[tk];
+        const int4 task = tasks[tk];
         const int inst = __builtin_amdgcn_readfirstlane(task.x);
-        const int local = __builtin_amdgcn_readfirstlane(task.y);
+        const int w = __builtin_amdgcn_readfirstlane(task.y);
+        const int r0 = __builtin_amdgcn_readfirstlane(task.z);
+        const int r1 = __builtin_amdgcn_readfirstlane(task.w);
         Ctl *c = ctls + inst;
-        if (!c->done) step_task<Exact>(descs[inst], c, local, lane);
+        const InstDesc &d = descs[inst];
+        // the tasks of an instance hold positions task_base .. task_base + ntask - 1
+        if (!c->done) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane);
         if (!persistent) break;
         unsigned int v = 0;
         if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
@@ -374,7 +378,7 @@ __global__ void fvm_ctl_start_kernel(const InstDesc *__restrict__ descs, Ctl *__
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int2 *tasks, int ntasks,
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream) {
     const dim3 block(wpb * kWave);
     if (exact)
