@@ -93,9 +93,11 @@ int fvm2d_download(fvm2d_solver *s, double *U5);
 int fvm2d_steps_total(const fvm2d_solver *s);
 int fvm2d_destroy(fvm2d_solver *s);
 
-/* Launch geometry of the step kernel.  strip_rows: output rows per wave;
- * waves_per_block: 1..4; persistent: 1 = persistent grid with dynamic task
- * tickets.  0 / negative values keep the current choice. */
+/* Launch geometry of the step kernel.  strip_rows: output rows per wave
+ * (0 keeps the current value); waves_per_block: 1..4 (0 keeps);
+ * persistent: 1 = persistent grid with dynamic task tickets, 0 = one wave
+ * per task, negative keeps the current choice.  Results do not depend on
+ * the geometry. */
 int fvm2d_set_launch(fvm2d_solver *s, int strip_rows, int waves_per_block, int persistent);
 int fvm2d_get_launch(const fvm2d_solver *s, int *strip_rows, int *waves_per_block,
                      int *persistent);
