@@ -559,6 +559,18 @@ struct Engine {
             for (int w : {2, 4}) {
                 for (int pers : {0, 1}) {
                     rc = set_launch(r, w, pers);
+                    if (rc) break;
+                    // One mesh in one round of two waves per SIMD: the task
+                    // order above pairs mirror-image tasks on a SIMD for the
+                    // dispatch of four waves per block; blocks of two waves
+                    // land on SIMDs in another pattern.  Trials from the
+                    // initial state cannot tell the two apart reliably
+                    // (2000 x 2000: blocks of two ahead by 2% over the first
+                    // 400 steps, behind by 4% over 1500), so only the
+                    // geometry the order was made for is tried.
+                    if (n == 1 && pers == 0 && w != 4 && ntasks > 4 * cu_count &&
+                        ntasks <= 8 * cu_count)
+                        continue;
                     if (!rc) rc = copy_states(false);
                     if (!rc) rc = begin();
                     if (!rc) rc = enqueue(2);
