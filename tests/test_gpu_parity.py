@@ -130,6 +130,28 @@ def test_launch_geometry_invariance(gpu, problems, mode):
             assert np.array_equal(U, base), (R, W, P)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_round_filling_strips_invariance(gpu, problems, mode):
+    """At 2000 x 2000 and 34-row strips the 2006 strips are re-cut into the
+    2048 strips of 33-34 rows that fill one round of wave slots; the result
+    equals that of plain 16-row strips and of the persistent scheduler
+    (bitwise)."""
+    U0 = problems.initialize_4shocks(2000)
+    base = None
+    for (R, W, P) in [(16, 4, 0), (34, 4, 0), (34, 4, 1)]:
+        s = gpu.Solver(2000, 2000, t_end=1.0, max_steps=4, mode=mode)
+        s.set_launch(R, W, P)
+        s.upload(U0)
+        steps, _, _ = s.run()
+        U = s.download()
+        s.close()
+        assert steps == 4
+        if base is None:
+            base = U
+        else:
+            assert np.array_equal(U, base), (R, W, P)
+
+
 def test_resume_like_reference(gpu, problems):
     """A second solve() continues from the previous result, t restarts at 0
     and the step count accumulates (src/wrappers.cuh:191,252,290-291)."""
