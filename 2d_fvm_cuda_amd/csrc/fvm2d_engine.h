@@ -57,6 +57,10 @@ inline int device_ok() {
 }
 
 // --------------------------------------------------------------------------
+// Row weight of the older waves' strips in a one-round launch of one mesh
+// (see set_launch).
+constexpr double kOlderWeight = 2.0;
+
 // Engine: n instances advanced by one step kernel per time step.
 struct Engine {
     int n = 0;
@@ -235,12 +239,33 @@ struct Engine {
                 const int base = (int)(T / d.nw), extra = (int)(T % d.nw);
                 std::vector<int> nsw(d.nw, base);
                 for (int e = 0; e < extra; e++) nsw[(e & 1) ? d.nw - 1 - e / 2 : e / 2]++;
+                // Strip heights.  In one round of two waves per SIMD the
+                // first 4 * cu_count tasks are the older waves, which take
+                // most issue slots while both run; the SIMD then runs its
+                // younger wave alone at a lower issue rate.  Giving the
+                // older waves' strips kOlderWeight times the rows of the
+                // younger waves' strips keeps both busy for longer
+                // (2000 x 2000 over 1500 steps: weight 1.6-2.4 all 2.2% less
+                // time per step than equal strips, 3.0 worse).  older[w]:
+                // strips of window w among the first 4 * cu_count tasks.
+                const long long S = 4LL * cu_count;
+                std::vector<int> older(d.nw, 0);
+                if (T == 2 * S) {
+                    long long idx = 0;
+                    for (int s = 0; s <= base && idx < S; s++)
+                        for (int w = 0; w < d.nw && idx < S; w++)
+                            if (s < nsw[w]) older[w]++, idx++;
+                }
+                auto bound = [&](int w, int s) {
+                    const int m = nsw[w], a = older[w];
+                    const double tot = a * kOlderWeight + (m - a);
+                    const double cum = s <= a ? s * kOlderWeight : a * kOlderWeight + (s - a);
+                    return 2 + (int)std::llround(H * cum / tot);
+                };
                 for (int s = 0; s <= base; s++)
                     for (int w = 0; w < d.nw; w++) {
-                        const int m = nsw[w];
-                        if (s >= m) continue;
-                        tasks.push_back(make_int4(i, w, 2 + (int)((long long)s * H / m),
-                                                  2 + (int)((long long)(s + 1) * H / m)));
+                        if (s >= nsw[w]) continue;
+                        tasks.push_back(make_int4(i, w, bound(w, s), bound(w, s + 1)));
                     }
             } else {
                 for (int s = 0; s < ns; s++)
