@@ -191,6 +191,48 @@ def test_batch_matches_plain(gpu, problems):
         b.close()
 
 
+def test_autotune_keeps_state_and_results(gpu, problems):
+    """fvm2d_autotune / fvm2d_batch_autotune time candidate launch shapes on
+    scratch copies: the resident state is unchanged afterwards, the chosen
+    shape is a valid one, and the run that follows equals a run with the
+    default shape (bitwise)."""
+    rng = problems.SplitMix64(23)
+    U0 = problems.random_four_quadrant(rng, 300, 211)
+    outs = []
+    for tune in (False, True):
+        s = gpu.Solver(300, 211, t_end=10.0, max_steps=5, mode=1)
+        s.upload(U0)
+        if tune:
+            best = s.autotune(steps_per_trial=2)
+            assert best > 0.0
+            R, W, P = s.get_launch()
+            assert R >= 1 and 1 <= W <= 4 and P in (0, 1)
+            assert np.array_equal(s.download(), U0)
+        steps, _, _ = s.run()
+        assert steps == 5
+        outs.append(s.download())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+
+    Us = [problems.random_four_quadrant(rng, nx, ny) for nx, ny in [(90, 70), (33, 120), (160, 48)]]
+    prms = [dict(t_end=10.0, max_steps=4, mode=1)] * len(Us)
+    res = []
+    for tune in (False, True):
+        b = gpu.Batch([U.shape[:2] for U in Us], prms)
+        for i, U in enumerate(Us):
+            b.upload(i, U)
+        if tune:
+            assert b.autotune(steps_per_trial=2) > 0.0
+            for i, U in enumerate(Us):
+                assert np.array_equal(b.download(i), U)
+        b.begin()
+        b.run(8)
+        res.append([b.download(i) for i in range(len(Us))])
+        b.close()
+    for a, c in zip(res[0], res[1]):
+        assert np.array_equal(a, c)
+
+
 def test_nan_state_reports_error(gpu, problems):
     U0 = problems.initialize_4shocks(32)
     U0[10, 10, 1] = -1.0  # negative pressure -> NaN sound speed
