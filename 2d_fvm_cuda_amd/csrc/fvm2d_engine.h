@@ -239,7 +239,9 @@ struct Engine {
                 const int base = (int)(T / d.nw), extra = (int)(T % d.nw);
                 std::vector<int> nsw(d.nw, base);
                 for (int e = 0; e < extra; e++) nsw[(e & 1) ? d.nw - 1 - e / 2 : e / 2]++;
-                // Strip heights.  In one round of two waves per SIMD the
+                // Strip heights.  In one round of two waves per SIMD
+                // launched four waves per block (the dispatch measured; two
+                // per block place the second round otherwise), the
                 // first 4 * cu_count tasks are the older waves, which take
                 // most issue slots while both run; the SIMD then runs its
                 // younger wave alone at a lower issue rate.  Giving the
@@ -250,7 +252,7 @@ struct Engine {
                 // strips of window w among the first 4 * cu_count tasks.
                 const long long S = 4LL * cu_count;
                 std::vector<int> older(d.nw, 0);
-                if (T == 2 * S) {
+                if (T == 2 * S && wpb == 4) {
                     long long idx = 0;
                     for (int s = 0; s <= base && idx < S; s++)
                         for (int w = 0; w < d.nw && idx < S; w++)
