@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
     }
 }
 
-constexpr int kSmaxBlocks = 256;  // blocks per instance of the s_max kernel
+constexpr int kSmaxBlocks = 256;  // most blocks per instance of the s_max kernel
 
 __device__ __forceinline__ void max_pair(unsigned long long &mx, long long &idx, unsigned long long m2,
                                          long long i2) {
@@ -213,7 +213,7 @@ __device__ __forceinline__ void wave_max_pair(unsigned long long &mx, long long 
 // s_max over every cell of U_n with the smallest linear index j * nx + i
 // attaining it (largest bit pattern, then smallest index: the result does
 // not depend on the reduction order), and dt = CFL h / s_max.  blockIdx.y is
-// the instance; each of its kSmaxBlocks blocks strides over rows and leaves
+// the instance; each of its gridDim.x blocks strides over rows and leaves
 // one pair, and the last block to arrive reduces the pairs.
 __global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ descs) {
     __shared__ unsigned long long smx[256 / kWave];
@@ -521,7 +521,12 @@ void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves
     hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(waves, 4)), dim3(256), 0, s, descs, tasks, ntasks);
 }
 void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
-    hipLaunchKernelGGL(adj_smax_kernel, dim3(kSmaxBlocks, ninst), dim3(256), 0, s, descs);
+    // about 1024 blocks over the whole batch: at 32 meshes of 256-768 rows,
+    // 256 blocks per mesh left each block a row or two and made the
+    // reduction tail dominate (backward 5 % longer than with 32 per mesh);
+    // a single 512 x 512 mesh is fastest with 256
+    const int per = std::max(32, std::min(kSmaxBlocks, 1024 / std::max(1, ninst)));
+    hipLaunchKernelGGL(adj_smax_kernel, dim3(per, ninst), dim3(256), 0, s, descs);
 }
 int adj_smax_pairs() { return kSmaxBlocks; }
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
