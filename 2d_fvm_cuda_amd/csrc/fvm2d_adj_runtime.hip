@@ -37,7 +37,7 @@ struct AdjEngine {
     // schedule
     std::vector<std::vector<long long>> cost;
     std::vector<std::vector<int>> split;
-    double *d_slot_t = nullptr;      // [slot * n + inst] time of the stored state
+    Ctl *d_slot_ctl = nullptr;       // [slot * n + inst] control block of the stored state
     AdjDesc *d_atab = nullptr;       // reverse-step descriptors, see build_desc_table
     std::vector<int> slot_step;      // step index held by each slot (-1: free)
     std::map<int, int> stored;       // step -> slot
@@ -135,7 +135,7 @@ struct AdjEngine {
         }
         rc = build_tasks();
         if (rc) return rc;
-        rc = dalloc(&d_slot_t, (size_t)slots * n);
+        rc = dalloc(&d_slot_ctl, (size_t)slots * n);
         if (!rc) rc = dalloc(&d_atab, (size_t)(slots + 2) * 2 * n);
         if (rc) return rc;
         build_schedule();
@@ -246,13 +246,13 @@ struct AdjEngine {
     }
 
     // Advance every instance by m >= 1 steps from the state in slot src,
-    // whose times are d_slot_t[src]; *res receives the engine buffer that
+    // whose control blocks are d_slot_ctl[src]; *res receives the engine buffer that
     // holds the result (kBuf0 or kBuf1: the time loop alternates them) and
     // the engine's control blocks its times.  Everything is queued on the
     // stream: the host does not wait.
     int advance(int src, int m, int *res) {
         int rc = copy_state(src, kBuf0);
-        if (!rc) rc = fwd.begin_device(d_slot_t + (size_t)src * n);
+        if (!rc) rc = fwd.begin_from_ctl(d_slot_ctl + (size_t)src * n);
         if (!rc) rc = fwd.enqueue(m);
         if (rc) return rc;
         fwd.begun = false;
@@ -261,13 +261,13 @@ struct AdjEngine {
         return FVM2D_OK;
     }
 
-    // Store the state at location src into slot s together with its times
-    // (the engine's control blocks after an advance).
+    // Store the state at location src into slot s together with the
+    // engine's control blocks after the advance that produced it.
     int store_slot(int src, int s) {
         int rc = copy_state(src, s);
         if (rc) return rc;
-        HIPCHK(hipMemcpy2DAsync(d_slot_t + (size_t)s * n, sizeof(double), &fwd.d_ctl[0].t, sizeof(Ctl),
-                                sizeof(double), n, hipMemcpyDeviceToDevice, fwd.stream));
+        HIPCHK(hipMemcpyAsync(d_slot_ctl + (size_t)s * n, fwd.d_ctl, n * sizeof(Ctl),
+                              hipMemcpyDeviceToDevice, fwd.stream));
         return FVM2D_OK;
     }
 
@@ -391,13 +391,22 @@ struct AdjEngine {
             if (rc) return rc;
             launch_chosen = true;
         }
-        HIPCHK(hipMemsetAsync(d_slot_t, 0, (size_t)slots * n * sizeof(double), fwd.stream));
         slot_step.assign(slots, -1);
         stored.clear();
         slot_step[0] = 0;
         stored[0] = 0;
         recompute_steps = 0;
         HIPCHK(hipEventRecord(r0, fwd.stream));
+        {
+            // control blocks of U_0 (t = 0, s_max and dt of the initial
+            // state) for the recomputations that start from slot 0
+            int rc = copy_state(0, kBuf0);
+            if (!rc) rc = fwd.begin_device(nullptr);
+            if (rc) return rc;
+            fwd.begun = false;
+            HIPCHK(hipMemcpyAsync(d_slot_ctl, fwd.d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToDevice,
+                                  fwd.stream));
+        }
         // walk the first spine of the schedule, storing its checkpoints
         int a = 0, len = nsteps, c = slots - 1, sa = 0;
         while (len > 1 && c > 0) {

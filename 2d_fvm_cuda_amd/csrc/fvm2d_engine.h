@@ -423,6 +423,16 @@ struct Engine {
         return FVM2D_OK;
     }
 
+    // Start time loops from control blocks saved after an earlier loop
+    // (device array of n), each instance's state already in buf0.  The
+    // arrival slots are zero: every completed launch clears them.
+    int begin_from_ctl(const Ctl *d_saved) {
+        launch_ctl_restore(d_ctl, d_saved, n, stream);
+        HIPCHK(hipGetLastError());
+        begun = true;
+        return FVM2D_OK;
+    }
+
     void launch_one() {
         launch_step(exact, d_desc, d_ctl, d_tasks, ntasks, d_sched, persistent, grid, wpb, stream);
     }

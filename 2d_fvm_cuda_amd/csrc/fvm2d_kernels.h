@@ -14,6 +14,7 @@ void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int us
                   long long nrows, hipStream_t stream);
 void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double *t0s,
                       hipStream_t stream);
+void launch_ctl_restore(Ctl *ctls, const Ctl *saved, int ninst, hipStream_t stream);
 int step_kernel_occupancy(bool exact, int wpb);
 
 }  // namespace fvm2d

@@ -377,6 +377,20 @@ __global__ void fvm_ctl_start_kernel(const InstDesc *__restrict__ descs, Ctl *__
     advance_ctl(d, c, t0s ? t0s[i] : 0.0, 0, sb);
 }
 
+// Resume time loops from saved control blocks (the state that goes with
+// each was saved beside it and is copied to buf0): t, dt and s_max as they
+// were, the step count restarted at 0 so that buf0 is the current buffer.
+__global__ void fvm_ctl_restore_kernel(Ctl *__restrict__ ctls, const Ctl *__restrict__ saved,
+                                       int ninst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    Ctl c = saved[i];
+    c.steps = 0;
+    c.smax_next = 0ull;
+    c.waves_done = 0u;
+    ctls[i] = c;
+}
+
 // ---------------------------------------------------------------- launchers
 void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream) {
@@ -406,6 +420,11 @@ void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double 
                       hipStream_t stream) {
     hipLaunchKernelGGL(fvm_ctl_start_kernel, dim3((ninst + 63) / 64), dim3(64), 0, stream, descs,
                        ctls, ninst, t0s);
+}
+
+void launch_ctl_restore(Ctl *ctls, const Ctl *saved, int ninst, hipStream_t stream) {
+    hipLaunchKernelGGL(fvm_ctl_restore_kernel, dim3((ninst + 63) / 64), dim3(64), 0, stream, ctls,
+                       saved, ninst);
 }
 
 int step_kernel_occupancy(bool exact, int wpb) {
