@@ -77,6 +77,7 @@ struct Engine {
     CtlSlot *d_slots = nullptr;      // n * kCtlSlots end-of-task arrival slots
     double *d_t0 = nullptr, *h_t0 = nullptr;  // per-instance start times
     int4 *d_tasks = nullptr;
+    std::vector<int4> h_tasks;       // host copy of d_tasks (source of its async upload)
     int ntasks = 0;
     int R = 16, wpb = 4, persistent = 0, grid = 0;
     int cu_count = 256;
@@ -213,7 +214,9 @@ struct Engine {
         if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
         if (persist >= 0) persistent = persist ? 1 : 0;
         if (R < 1) return fail(FVM2D_ERR_ARG, "strip rows must be >= 1");
-        std::vector<int4> tasks;
+        // host copy kept as a member: the upload below is asynchronous
+        std::vector<int4> &tasks = h_tasks;
+        tasks.clear();
         // resident waves of the launch: blocks per CU x waves per block x CUs
         const long long slots =
             (long long)std::max(1, step_kernel_occupancy(exact, wpb)) * wpb * cu_count;
