@@ -195,11 +195,22 @@ struct AdjEngine {
         if ((rc = dalloc(&d_tr3a, nr3a))) return rc;
         if ((rc = dalloc(&d_tr3b, nr3b))) return rc;
         for (int i = 0; i < n; i++) had[i].dtpart = dtpart;
-        HIPCHK(hipMemcpyAsync(d_tfwd, tf.data(), nfwd * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
-        HIPCHK(hipMemcpyAsync(d_tr2, t2.data(), nr2 * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
-        HIPCHK(hipMemcpyAsync(d_tr3a, t3a.data(), nr3a * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
-        HIPCHK(hipMemcpyAsync(d_tr3b, t3b.data(), nr3b * sizeof(int4), hipMemcpyHostToDevice, fwd.stream));
-        HIPCHK(hipStreamSynchronize(fwd.stream));
+        // the host vectors are locals: wait for the copies on every path
+        hipError_t e = hipMemcpyAsync(d_tfwd, tf.data(), nfwd * sizeof(int4), hipMemcpyHostToDevice,
+                                      fwd.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tr2, t2.data(), nr2 * sizeof(int4), hipMemcpyHostToDevice, fwd.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tr3a, t3a.data(), nr3a * sizeof(int4), hipMemcpyHostToDevice,
+                               fwd.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tr3b, t3b.data(), nr3b * sizeof(int4), hipMemcpyHostToDevice,
+                               fwd.stream);
+        const hipError_t es = hipStreamSynchronize(fwd.stream);
+        if (e == hipSuccess) e = es;
+        if (e != hipSuccess)
+            return fail(FVM2D_ERR_HIP, std::string("uploading the adjoint task lists: ") +
+                                           hipGetErrorString(e));
         return FVM2D_OK;
     }
 
