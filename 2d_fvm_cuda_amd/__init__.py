@@ -67,7 +67,7 @@ EXPORTS = [
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
-    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe", "fvm2d_recip_probe",
+    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_strip_layout", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe", "fvm2d_recip_probe",
     "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
     "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
@@ -118,6 +118,9 @@ def lib():
         "fvm2d_batch_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "fvm2d_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "fvm2d_batch_destroy": (ctypes.c_int, [_vp]),
+        "fvm2d_strip_layout": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_int),
+                                                                 ctypes.c_int,
+                                                                 ctypes.POINTER(ctypes.c_int)]),
         "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
         "fvm2d_valu_probe": (ctypes.c_int, [ctypes.c_int, _dp]),
         "fvm2d_recip_probe": (ctypes.c_int, [_dp]),
@@ -161,6 +164,21 @@ def recip_probe():
 
 def device_available() -> bool:
     return bool(lib().fvm2d_device_available())
+
+
+def strip_layout(nx: int, ny: int, strip_rows: int, waves_per_block: int = 4, persistent: int = 0,
+                 cu_count: int = 256, blocks_per_cu: int = 2) -> np.ndarray:
+    """The strips one launch assigns to its waves, in launch position order:
+    an (ntasks, 3) array of (column window, first row, end row).  Needs no
+    device."""
+    n = ctypes.c_int(0)
+    check(lib().fvm2d_strip_layout(nx, ny, strip_rows, waves_per_block, persistent, cu_count,
+                                   blocks_per_cu, None, 0, ctypes.byref(n)), "fvm2d_strip_layout")
+    out = np.zeros((n.value, 3), dtype=np.int32)
+    check(lib().fvm2d_strip_layout(nx, ny, strip_rows, waves_per_block, persistent, cu_count,
+                                   blocks_per_cu, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   n.value, ctypes.byref(n)), "fvm2d_strip_layout")
+    return out
 
 
 def hbm_copy_probe(n_doubles: int, reps: int) -> float:

@@ -209,6 +209,76 @@ struct Engine {
         return std::max(8, std::min(64, r));
     }
 
+    // Strips of one mesh (instance inst: nw column windows, ny rows with
+    // ghosts) for strip height R, appended to tasks as {inst, window, first
+    // row, end row}.  single_fixed: the launch holds this mesh alone with
+    // one wave per task; slots: resident waves of the launch; S: SIMDs
+    // (4 * CUs).  No device state is used.
+    static void mesh_strips(int inst, int nw, int ny, int R, bool single_fixed, long long slots,
+                            long long S, int wpb, std::vector<int4> &tasks) {
+        const int i = inst;
+        const int H = ny - 4;
+        const int ns = (H + R - 1) / R;
+        // One mesh whose strips of R rows leave the last round of
+        // resident waves partly empty (at 2000 x 2000 and R = 34: 2006
+        // tasks for 2048 wave slots, so ten CUs run half their waves):
+        // cut the windows into more, slightly lower strips so that the
+        // task count fills the round exactly.  Windows 0, nw-1, 1,
+        // nw-2, ... take the extra strips (edge windows also write the
+        // ghost cells).  The cut decides only which wave computes
+        // which rows.
+        const long long T0 = (long long)nw * ns;
+        const long long k = (T0 + slots - 1) / slots;
+        const long long T = k * slots;
+        if (single_fixed && k <= 2 && T0 < T && T0 * 10 >= T * 8 &&
+            T <= (long long)nw * (H / 4)) {
+            const int base = (int)(T / nw), extra = (int)(T % nw);
+            std::vector<int> nsw(nw, base);
+            for (int e = 0; e < extra; e++) nsw[(e & 1) ? nw - 1 - e / 2 : e / 2]++;
+            // Strip heights.  In one round of two waves per SIMD
+            // launched four waves per block (the dispatch measured; two
+            // per block place the second round otherwise), the
+            // first S tasks are the older waves, which take
+            // most issue slots while both run; the SIMD then runs its
+            // younger wave alone at a lower issue rate.  Giving the
+            // older waves' strips kOlderWeight times the rows of the
+            // younger waves' strips keeps both busy for longer
+            // (2000 x 2000 over 1500 steps: weight 1.6-2.4 all 2.2% less
+            // time per step than equal strips, 3.0 worse).  older[w]:
+            // strips of window w among the first S tasks.
+            std::vector<int> older(nw, 0);
+            if (T == 2 * S && wpb == 4) {
+                long long idx = 0;
+                for (int s = 0; s <= base && idx < S; s++)
+                    for (int w = 0; w < nw && idx < S; w++)
+                        if (s < nsw[w]) older[w]++, idx++;
+            }
+            auto bound = [&](int w, int s) {
+                const int m = nsw[w], a = older[w];
+                const double tot = a * kOlderWeight + (m - a);
+                const double cum = s <= a ? s * kOlderWeight : a * kOlderWeight + (s - a);
+                return 2 + (int)std::llround(H * cum / tot);
+            };
+            for (int s = 0; s <= base; s++)
+                for (int w = 0; w < nw; w++) {
+                    if (s >= nsw[w]) continue;
+                    tasks.push_back(make_int4(i, w, bound(w, s), bound(w, s + 1)));
+                }
+        } else {
+            for (int s = 0; s < ns; s++)
+                for (int w = 0; w < nw; w++) {
+                    const int r0 = 2 + s * R;
+                    tasks.push_back(make_int4(i, w, r0, std::min(ny - 2, r0 + R)));
+                }
+        }
+    }
+
+    // The second round of one mesh's tasks in reverse order (see set_launch).
+    static void mirror_second_round(std::vector<int4> &tasks, long long S) {
+        const long long nt = (long long)tasks.size();
+        if (nt > S && nt <= 2 * S) std::reverse(tasks.begin() + S, tasks.end());
+    }
+
     int set_launch(int strip_rows, int waves_per_block, int persist) {
         if (strip_rows > 0) R = strip_rows;
         if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
@@ -224,61 +294,7 @@ struct Engine {
             InstDesc &d = hd[i];
             d.R = R;
             d.task_base = (int)tasks.size();
-            const int H = d.ny - 4;
-            const int ns = (H + R - 1) / R;
-            // One mesh whose strips of R rows leave the last round of
-            // resident waves partly empty (at 2000 x 2000 and R = 34: 2006
-            // tasks for 2048 wave slots, so ten CUs run half their waves):
-            // cut the windows into more, slightly lower strips so that the
-            // task count fills the round exactly.  Windows 0, nw-1, 1,
-            // nw-2, ... take the extra strips (edge windows also write the
-            // ghost cells).  The cut decides only which wave computes
-            // which rows.
-            const long long T0 = (long long)d.nw * ns;
-            const long long k = (T0 + slots - 1) / slots;
-            const long long T = k * slots;
-            if (n == 1 && !persistent && k <= 2 && T0 < T && T0 * 10 >= T * 8 &&
-                T <= (long long)d.nw * (H / 4)) {
-                const int base = (int)(T / d.nw), extra = (int)(T % d.nw);
-                std::vector<int> nsw(d.nw, base);
-                for (int e = 0; e < extra; e++) nsw[(e & 1) ? d.nw - 1 - e / 2 : e / 2]++;
-                // Strip heights.  In one round of two waves per SIMD
-                // launched four waves per block (the dispatch measured; two
-                // per block place the second round otherwise), the
-                // first 4 * cu_count tasks are the older waves, which take
-                // most issue slots while both run; the SIMD then runs its
-                // younger wave alone at a lower issue rate.  Giving the
-                // older waves' strips kOlderWeight times the rows of the
-                // younger waves' strips keeps both busy for longer
-                // (2000 x 2000 over 1500 steps: weight 1.6-2.4 all 2.2% less
-                // time per step than equal strips, 3.0 worse).  older[w]:
-                // strips of window w among the first 4 * cu_count tasks.
-                const long long S = 4LL * cu_count;
-                std::vector<int> older(d.nw, 0);
-                if (T == 2 * S && wpb == 4) {
-                    long long idx = 0;
-                    for (int s = 0; s <= base && idx < S; s++)
-                        for (int w = 0; w < d.nw && idx < S; w++)
-                            if (s < nsw[w]) older[w]++, idx++;
-                }
-                auto bound = [&](int w, int s) {
-                    const int m = nsw[w], a = older[w];
-                    const double tot = a * kOlderWeight + (m - a);
-                    const double cum = s <= a ? s * kOlderWeight : a * kOlderWeight + (s - a);
-                    return 2 + (int)std::llround(H * cum / tot);
-                };
-                for (int s = 0; s <= base; s++)
-                    for (int w = 0; w < d.nw; w++) {
-                        if (s >= nsw[w]) continue;
-                        tasks.push_back(make_int4(i, w, bound(w, s), bound(w, s + 1)));
-                    }
-            } else {
-                for (int s = 0; s < ns; s++)
-                    for (int w = 0; w < d.nw; w++) {
-                        const int r0 = 2 + s * R;
-                        tasks.push_back(make_int4(i, w, r0, std::min(d.ny - 2, r0 + R)));
-                    }
-            }
+            mesh_strips(i, d.nw, d.ny, R, n == 1 && !persistent, slots, 4LL * cu_count, wpb, tasks);
             d.ntask = (int)tasks.size() - d.task_base;
         }
         ntasks = (int)tasks.size();
@@ -294,10 +310,7 @@ struct Engine {
         // Reversing the second round pairs each task with its mirror image
         // through the mesh centre instead (2.5% less time per 2000^2 step).
         // The order decides only which wave computes which task.
-        if (n == 1 && !persistent) {
-            const int S = 4 * cu_count;
-            if (ntasks > S && ntasks <= 2 * S) std::reverse(tasks.begin() + S, tasks.end());
-        }
+        if (n == 1 && !persistent) mirror_second_round(tasks, 4LL * cu_count);
         if (d_tasks) HIPCHK(hipFree(d_tasks));
         d_tasks = nullptr;
         HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int4)));

@@ -330,6 +330,27 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(const double2 *__restrict
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += stride) dst[k] = src[k];
 }
 
+int fvm2d_strip_layout(int nx, int ny, int strip_rows, int waves_per_block, int persistent,
+                       int cu_count, int blocks_per_cu, int *out, int max_tasks, int *ntasks_out) {
+    if (nx < 5 || ny < 5 || strip_rows < 1 || waves_per_block < 1 || waves_per_block > 4 ||
+        cu_count < 1 || blocks_per_cu < 1 || max_tasks < 0 || (max_tasks > 0 && !out) ||
+        !ntasks_out)
+        return fail(FVM2D_ERR_ARG, "bad layout query");
+    const int nw = (nx - 4 + kOwn - 1) / kOwn;
+    const long long slots = (long long)blocks_per_cu * waves_per_block * cu_count;
+    std::vector<int4> tasks;
+    Engine::mesh_strips(0, nw, ny, strip_rows, persistent == 0, slots, 4LL * cu_count,
+                        waves_per_block, tasks);
+    if (persistent == 0) Engine::mirror_second_round(tasks, 4LL * cu_count);
+    *ntasks_out = (int)tasks.size();
+    for (int k = 0; k < (int)tasks.size() && k < max_tasks; k++) {
+        out[3 * k] = tasks[k].y;
+        out[3 * k + 1] = tasks[k].z;
+        out[3 * k + 2] = tasks[k].w;
+    }
+    return FVM2D_OK;
+}
+
 int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out) {
     if (n_doubles < 2 || (n_doubles & 1) || reps < 1) return fail(FVM2D_ERR_ARG, "need an even n_doubles >= 2 and reps >= 1");
     if (int rc0 = device_ok()) return rc0;

@@ -137,6 +137,15 @@ int fvm2d_batch_set_launch(fvm2d_batch *b, int strip_rows, int waves_per_block, 
 int fvm2d_batch_autotune(fvm2d_batch *b, int steps_per_trial, double *best_ms_out);
 int fvm2d_batch_destroy(fvm2d_batch *b);
 
+/* Diagnostic with no counterpart in the reference: the strips a launch of
+ * one nx x ny mesh assigns to its waves, in launch position order, as
+ * (column window, first row, end row) triples in out[3 k .. 3 k + 2] for
+ * k < min(*ntasks_out, max_tasks).  cu_count and blocks_per_cu (resident
+ * blocks of waves_per_block waves per CU) describe the device; no device
+ * is used. */
+int fvm2d_strip_layout(int nx, int ny, int strip_rows, int waves_per_block, int persistent,
+                       int cu_count, int blocks_per_cu, int *out, int max_tasks, int *ntasks_out);
+
 /* Diagnostic with no counterpart in the reference: reps launches of a plain
  * device-to-device copy of n_doubles FP64 values (16-byte accesses, two
  * buffers allocated and freed by the call).  Its FETCH_SIZE / WRITE_SIZE
