@@ -284,6 +284,14 @@ struct Engine {
         if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
         if (persist >= 0) persistent = persist ? 1 : 0;
         if (R < 1) return fail(FVM2D_ERR_ARG, "strip rows must be >= 1");
+        {
+            // task positions are ints: refuse geometries with more tasks
+            // (the round-filling cut at most doubles a mesh's count)
+            long long total = 0;
+            for (int i = 0; i < n; i++)
+                total += 2LL * hd[i].nw * ((hd[i].ny - 4 + R - 1) / R) + 1;
+            if (total > (1LL << 30)) return fail(FVM2D_ERR_ARG, "strip rows too low for this mesh");
+        }
         // host copy kept as a member: the upload below is asynchronous
         std::vector<int4> &tasks = h_tasks;
         tasks.clear();

@@ -337,6 +337,8 @@ int fvm2d_strip_layout(int nx, int ny, int strip_rows, int waves_per_block, int 
         !ntasks_out)
         return fail(FVM2D_ERR_ARG, "bad layout query");
     const int nw = (nx - 4 + kOwn - 1) / kOwn;
+    if (2LL * nw * ((ny - 4 + strip_rows - 1) / strip_rows) > (1LL << 30))
+        return fail(FVM2D_ERR_ARG, "strip rows too low for this mesh");
     const long long slots = (long long)blocks_per_cu * waves_per_block * cu_count;
     std::vector<int4> tasks;
     Engine::mesh_strips(0, nw, ny, strip_rows, persistent == 0, slots, 4LL * cu_count,

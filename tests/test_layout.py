@@ -77,3 +77,5 @@ def test_bad_query_rejected(lay):
         lay.strip_layout(2000, 2000, 0)
     with pytest.raises(lay.FVM2DError):
         lay.strip_layout(2000, 2000, 34, 5)
+    with pytest.raises(lay.FVM2DError, match="too low"):
+        lay.strip_layout(200000, 200000, 1)   # more tasks than int positions hold
