@@ -280,18 +280,19 @@ struct Engine {
     }
 
     int set_launch(int strip_rows, int waves_per_block, int persist) {
-        if (strip_rows > 0) R = strip_rows;
-        if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
-        if (persist >= 0) persistent = persist ? 1 : 0;
-        if (R < 1) return fail(FVM2D_ERR_ARG, "strip rows must be >= 1");
+        const int newR = strip_rows > 0 ? strip_rows : R;
         {
             // task positions are ints: refuse geometries with more tasks
-            // (the round-filling cut at most doubles a mesh's count)
+            // (the round-filling cut at most doubles a mesh's count); the
+            // current geometry stays as it was
             long long total = 0;
             for (int i = 0; i < n; i++)
-                total += 2LL * hd[i].nw * ((hd[i].ny - 4 + R - 1) / R) + 1;
+                total += 2LL * hd[i].nw * ((hd[i].ny - 4 + newR - 1) / newR) + 1;
             if (total > (1LL << 30)) return fail(FVM2D_ERR_ARG, "strip rows too low for this mesh");
         }
+        R = newR;
+        if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
+        if (persist >= 0) persistent = persist ? 1 : 0;
         // host copy kept as a member: the upload below is asynchronous
         std::vector<int4> &tasks = h_tasks;
         tasks.clear();
