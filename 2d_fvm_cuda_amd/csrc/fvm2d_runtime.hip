@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(const double2 *__restrict
 
 int fvm2d_strip_layout(int nx, int ny, int strip_rows, int waves_per_block, int persistent,
                        int cu_count, int blocks_per_cu, int *out, int max_tasks, int *ntasks_out) {
-    if (nx < 5 || ny < 5 || strip_rows < 1 || waves_per_block < 1 || waves_per_block > 4 ||
+    if (nx < 6 || ny < 6 || strip_rows < 1 || waves_per_block < 1 || waves_per_block > 4 ||
         cu_count < 1 || blocks_per_cu < 1 || max_tasks < 0 || (max_tasks > 0 && !out) ||
         !ntasks_out)
         return fail(FVM2D_ERR_ARG, "bad layout query");

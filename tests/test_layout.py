@@ -77,5 +77,8 @@ def test_bad_query_rejected(lay):
         lay.strip_layout(2000, 2000, 0)
     with pytest.raises(lay.FVM2DError):
         lay.strip_layout(2000, 2000, 34, 5)
+    with pytest.raises(lay.FVM2DError):
+        lay.strip_layout(5, 64, 8)          # below the 6 x 6 minimum of a mesh
+    assert lay.strip_layout(6, 6, 1).tolist() == [[0, 2, 3], [0, 3, 4]]
     with pytest.raises(lay.FVM2DError, match="too low"):
         lay.strip_layout(200000, 200000, 1)   # more tasks than int positions hold
