@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfvm2d.so")
 
 SOURCES = ["fvm2d_step.hip", "fvm2d_adjoint.hip", "fvm2d_runtime.hip", "fvm2d_adj_runtime.hip"]
-HEADERS = ["fvm2d_cell.h", "fvm2d_device.h", "fvm2d_kernels.h", "fvm2d_adjoint_cell.h"]
+HEADERS = ["fvm2d_cell.h", "fvm2d_device.h", "fvm2d_kernels.h", "fvm2d_adjoint_cell.h",
+           "fvm2d_march.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -30,8 +30,10 @@ struct AdjEngine {
     double *dtpart = nullptr;
     AdjCtl *d_actl = nullptr;
     std::vector<AdjDesc> had;
-    int4 *d_tfwd = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
-    int nfwd = 0, nr2 = 0, nr3a = 0, nr3b = 0;
+    int4 *d_tfwd = nullptr, *d_tfy = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
+    int nfwd = 0, nfy = 0, nr2 = 0, nr3a = 0, nr3b = 0;
+    // rows per task of the marched B1 = Y(U) recompute
+    static constexpr int kFwdYRows = 16;
     std::vector<void *> allocs;
     std::vector<double *> stage_in, stage_out;  // host-pointer API staging
     // schedule
@@ -144,16 +146,32 @@ struct AdjEngine {
     }
 
     int build_tasks() {
-        std::vector<int4> tf, t2, t3a, t3b;
+        std::vector<int4> tf, tfy, t2, t3a, t3b;
+        // B1 = Y(U) by the register march (strips of kFwdYRows interior
+        // rows of 64 whole columns: row-segment loads) when its strips fill
+        // two waves per SIMD; otherwise by column lines like A1 (more, smaller
+        // tasks: at 512 x 512 the march has 256 strips and its reverse step
+        // took 5 % longer; on the 32-mesh batch it saves 7 % of the backward)
+        long long strips = 0;
+        for (int i = 0; i < n; i++)
+            strips += (long long)((nx[i] + kWave - 1) / kWave) *
+                      ((ny[i] - 4 + kFwdYRows - 1) / kFwdYRows);
+        const bool march = strips >= 2LL * 4 * fwd.cu_count;
         for (int i = 0; i < n; i++) {
             const int wf_x = (nx[i] + 59) / 60, wf_y = (ny[i] + 59) / 60;
-            const int wa_x = (nx[i] + 55) / 56, wa_y = (ny[i] + 55) / 56;
             for (int j = 0; j < ny[i]; j++)
                 for (int w = 0; w < wf_x; w++) tf.push_back(make_int4(i, 0, w, j));
-            // Y tasks: adjacent columns consecutive, so the waves of a block
-            // (and of neighbouring blocks) share the cache lines of each row
-            for (int w = 0; w < wf_y; w++)
-                for (int c = 0; c < nx[i]; c++) tf.push_back(make_int4(i, 1, w, c));
+            if (march) {
+                const int H = ny[i] - 4, wy = (nx[i] + kWave - 1) / kWave;
+                for (int r0 = 2; r0 < 2 + H; r0 += kFwdYRows)
+                    for (int w = 0; w < wy; w++)
+                        tfy.push_back(make_int4(i, w, r0, std::min(2 + H, r0 + kFwdYRows)));
+            } else {
+                // Y lines: adjacent columns consecutive, so the waves of a
+                // block (and of neighbouring blocks) share each row's lines
+                for (int w = 0; w < wf_y; w++)
+                    for (int c = 0; c < nx[i]; c++) tf.push_back(make_int4(i, 1, w, c));
+            }
         }
         int base = 0;
         for (int i = 0; i < n; i++) {  // R2: X^T on B1 (rows) and Y^T on A1 (columns)
@@ -185,12 +203,14 @@ struct AdjEngine {
         }
         base += (int)t3b.size();
         nfwd = (int)tf.size();
+        nfy = (int)tfy.size();
         nr2 = (int)t2.size();
         nr3a = (int)t3a.size();
         nr3b = (int)t3b.size();
         int rc;
         if ((rc = dalloc(&dtpart, base))) return rc;
         if ((rc = dalloc(&d_tfwd, nfwd))) return rc;
+        if ((rc = dalloc(&d_tfy, nfy))) return rc;
         if ((rc = dalloc(&d_tr2, nr2))) return rc;
         if ((rc = dalloc(&d_tr3a, nr3a))) return rc;
         if ((rc = dalloc(&d_tr3b, nr3b))) return rc;
@@ -198,6 +218,8 @@ struct AdjEngine {
         // the host vectors are locals: wait for the copies on every path
         hipError_t e = hipMemcpyAsync(d_tfwd, tf.data(), nfwd * sizeof(int4), hipMemcpyHostToDevice,
                                       fwd.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tfy, tfy.data(), nfy * sizeof(int4), hipMemcpyHostToDevice, fwd.stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_tr2, t2.data(), nr2 * sizeof(int4), hipMemcpyHostToDevice, fwd.stream);
         if (e == hipSuccess)
@@ -337,6 +359,7 @@ struct AdjEngine {
         // registers), four for the lighter forward recompute
         const int simds = 4 * fwd.cu_count;
         adj_launch_fwd(descs, d_tfwd, nfwd, 4 * simds, fwd.stream);
+        adj_launch_fwd_y(descs, d_tfy, nfy, fwd.stream);
         adj_launch_sweep(descs, d_tr2, nr2, 0, 0, simds, fwd.stream);
         adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, simds, fwd.stream);
         adj_launch_sweep(descs, d_tr3b, nr3b, 2, nr2 + nr3a, simds, fwd.stream);

@@ -22,6 +22,7 @@
 #include "fvm2d_adjoint_kernels.h"
 #include "fvm2d_ghost.h"
 #include "fvm2d_lanes.h"
+#include "fvm2d_march.h"
 
 namespace fvm2d {
 
@@ -192,6 +193,61 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
         t = t2;
         m = m2;
         q = q2;
+    }
+}
+
+// Forward first sweep B1 = Y(U) with the step kernel's register march: a
+// wave owns 64 whole columns (the Y-sweep is column-local, no halo) and a
+// strip of interior rows [r0, r1) (task {instance, 64-column window, r0,
+// r1}); it reads rows r0-2 .. r1+1 along the columns, so every load and
+// store is a row segment.  Rows 0, 1, ny-2 and ny-1 are not swept (the
+// reference updates lines [2, n-3]): the first and last strips copy them
+// from U.  Same operations on the same values as the line form above.
+__global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ descs,
+                                                        const int4 *__restrict__ tasks, int ntasks) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+    if (tk >= ntasks) return;
+    const int4 t = uniform_task(tasks, tk);
+    const AdjDesc &d = descs[t.x];
+    const int nx = d.nx, ny = d.ny, r0 = t.z, r1 = t.w;
+    const long long P = d.plane, pitch = d.pitch;
+    const double *__restrict__ U = d.U;
+    double *__restrict__ B1 = d.B1;
+    const int col = t.y * kWave + lane;
+    const int colc = col < nx ? col : nx - 1;
+    const bool valid = col < nx;
+    const double dt = d.ctl->dt;
+    const Line LY = d.ly;
+    const double lamy = dt / LY.h;
+    auto load = [&](int j) -> Q {
+        const long long o = (long long)j * pitch + colc;
+        return Q{U[o], U[P + o], U[2 * P + o], U[3 * P + o]};
+    };
+    auto store = [&](int j, const Q &q) {
+        if (!valid) return;
+        const long long o = (long long)j * pitch + col;
+        B1[o] = q.r;
+        B1[P + o] = q.p;
+        B1[2 * P + o] = q.un;
+        B1[3 * P + o] = q.ut;
+    };
+    if (r0 == 2) {
+        store(0, load(0));
+        store(1, load(1));
+    }
+    if (r1 == ny - 2) {
+        store(ny - 2, load(ny - 2));
+        store(ny - 1, load(ny - 1));
+    }
+    YMarch mu;
+    ymarch_init<false>(mu, swap_uv(load(r0 - 2)), swap_uv(load(r0 - 1)), swap_uv(load(r0)),
+                       swap_uv(load(r0 + 1)), LY, d.gas, dt);
+    Q u_n = load(r0 + 2);
+    for (int j = r0; j < r1; j++) {
+        const Q u_cur = u_n;
+        if (j + 1 < r1) u_n = load(j + 3);
+        store(j, swap_uv(ymarch_step<false>(mu, swap_uv(u_cur), LY, d.gas, dt, lamy)));
     }
 }
 
@@ -519,6 +575,11 @@ static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n +
 void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves, hipStream_t s) {
     const int waves = ntasks < max_waves ? ntasks : max_waves;
     hipLaunchKernelGGL(adj_fwd_kernel, dim3(blocks_for(waves, 4)), dim3(256), 0, s, descs, tasks, ntasks);
+}
+void adj_launch_fwd_y(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s) {
+    if (ntasks > 0)
+        hipLaunchKernelGGL(adj_fwd_y_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs,
+                           tasks, ntasks);
 }
 void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
     // about 1024 blocks over the whole batch: at 32 meshes of 256-768 rows,

@@ -39,6 +39,9 @@ struct AdjDesc {
 
 // max_waves: the waves of a launch (each walks a strided task list).
 void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves, hipStream_t s);
+// B1 = Y(U) by the register march; tasks {instance, 64-column window, r0, r1}
+// tile every instance's interior rows [2, ny-2) for every window.
+void adj_launch_fwd_y(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s);
 // s_max, its arg-max and dt of every instance (one launch); the pair arrays
 // hold adj_smax_pairs() entries per instance.
 void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s);
