@@ -312,7 +312,7 @@ class Solver:
 
 
 class Batch:
-    """Batched forward: many meshes advanced by one persistent launch per step."""
+    """Batched forward: many meshes advanced by one launch per step."""
 
     def __init__(self, shapes, params_list):
         n = len(shapes)

@@ -1,7 +1,7 @@
 """Benchmark legs beyond the plain 2000^2 solve (used by bench.py).
 
 batched()          64 seeded four-quadrant meshes, sides ~U{128..1024}, 200 steps
-                   each in one persistent launch per step (hipGraph-captured),
+                   all in one launch per step (hipGraph-captured),
                    against the same meshes solved one after another.
 adjoint()          512^2 Fourier-perturbed state, 400 steps forward with
                    binomial checkpointing, then the reverse sweep; reports the

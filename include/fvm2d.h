@@ -120,7 +120,8 @@ int fvm2d_time_kernel(fvm2d_solver *s, int nsteps, double *avg_ms_out, double *m
 
 /* ---- batched forward (no counterpart in the reference) ----------------- *
  * n independent meshes of their own size and parameters advance together:
- * one persistent launch per time step covers every instance's tiles, each
+ * one launch per time step covers every instance's tiles (persistent waves
+ * taking task tickets, or one wave per task: the autotuner chooses), each
  * instance keeps its own t, dt and step counter on the device, and the
  * step loop is captured in a hipGraph. */
 typedef struct fvm2d_batch fvm2d_batch;
