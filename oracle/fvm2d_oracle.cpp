@@ -5,12 +5,20 @@
 // -ffp-contract=off, every result is bit-reproducible.
 //
 // Pinned by tests/test_oracle_pins.py: flux, limiter, cons/prim, wave speed,
-// the GRP's supersonic branches and star values (Suliciu relaxation
-// solution), its continuous-data limit, mirror symmetry, the ghost presets,
-// the rotation and the whole-solve invariants.  Parity unpinned: the
-// star-region time derivatives for discontinuous data
-// (src/basicFuns.cuh:151-185) beyond that limit and those identities; the
-// reference cannot be built here and ships no tests or vectors for them.
+// the ghost presets, the rotation and the whole-solve invariants; the GRP's
+// star values (Suliciu relaxation solution) and its time derivatives for
+// discontinuous data: zero for piecewise-constant data on every branch,
+// linear in the slopes, invariant under the Euler scaling laws (bitwise for
+// powers of two), the linear-acoustics interface solution at u* = 0 with
+// unequal impedances, the continuous-data limit, and, on every branch, the
+// characteristic solution of the Lagrangian relaxation system wherever the
+// slopes leave the impedance unchanged (p rho_x + rho p_x = 0).  Parity
+// unpinned: only the coefficients of the star-region terms proportional to
+// the impedance slope c_x at u* != u_K (src/basicFuns.cuh:151-154, 165,
+// 168, 179, 182); they are the author's modelling choices (an exact
+// variable-impedance relaxation solution has other coefficients), so only a
+// compiled reference would fix them, and the restatement follows the source
+// token for token there.
 #include "fvm2d_oracle.h"
 
 #include <algorithm>

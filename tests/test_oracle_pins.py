@@ -426,3 +426,247 @@ def test_step_count_config1(orc, problems):
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1_256.json")))
     assert fx["steps"] == n
     assert hashlib.sha256(np.ascontiguousarray(U[..., :4]).tobytes()).hexdigest() == fx["sha256_prim4"]
+
+
+# ----------------------------------------------------------------- GRP time derivatives
+# The pins below fix the GRP's interface time derivatives for discontinuous
+# data (src/basicFuns.cuh:129-185).  Branch labels: 0 left supersonic,
+# 1 right supersonic, 2 star region with u* <= 0, 3 star region with u* > 0.
+
+def _rand_state(rng, umax):
+    return np.array([rng.uniform(0.2, 3), rng.uniform(0.2, 3), rng.uniform(-umax, umax),
+                     rng.normal()])
+
+
+def _grp_bins(orc, rng, per_bin, slope_fn=None):
+    """Random interface problems, per_bin of each (branch, p_R >= p_L) pair:
+    all four branches under both pressure orderings."""
+    bins = {(b, o): [] for b in range(4) for o in (False, True)}
+    tries = 0
+    while any(len(v) < per_bin for v in bins.values()):
+        tries += 1
+        assert tries < 200000, "sampler cannot reach every branch"
+        umax = rng.choice([1.5, 6.0])
+        WL, WR = _rand_state(rng, umax), _rand_state(rng, umax)
+        if slope_fn is None:
+            SL, SR = rng.normal(size=4), rng.normal(size=4)
+        else:
+            SL, SR = slope_fn(rng, WL), slope_fn(rng, WR)
+        _, _, br = orc.grp(WL, WR, SL, SR, G)
+        key = (br, bool(WR[1] >= WL[1]))
+        if len(bins[key]) < per_bin:
+            bins[key].append((WL, WR, SL, SR))
+    return bins
+
+
+def test_grp_zero_slopes_give_zero_time_derivatives(orc):
+    """(a) Piecewise-constant data: every branch returns W_t = 0 for any
+    jump, under both pressure orderings (the relaxation Riemann solution is
+    self-similar, so the interface state does not change in time)."""
+    rng = np.random.default_rng(101)
+    z = np.zeros(4)
+    for key, cases in _grp_bins(orc, rng, 40).items():
+        for WL, WR, _, _ in cases:
+            W, Wt, br = orc.grp(WL, WR, z, z, G)
+            assert br == key[0]
+            np.testing.assert_array_equal(Wt, 0.0)
+
+
+def test_grp_time_derivatives_linear_in_slopes(orc):
+    """(b) For fixed face states the branch, the relaxation speeds and the
+    star values do not depend on the slopes, and W_t is linear in the 16
+    slope inputs: W_t(a S1 + b S2) = a W_t(S1) + b W_t(S2), and the 5 x 16
+    matrix assembled from unit slopes reproduces W_t for random slopes."""
+    rng = np.random.default_rng(102)
+    for key, cases in _grp_bins(orc, rng, 25).items():
+        for WL, WR, SL, SR in cases:
+            W0, _, _ = orc.grp(WL, WR, np.zeros(4), np.zeros(4), G)
+            J = np.zeros((5, 8))
+            for k in range(8):
+                e = np.zeros(8)
+                e[k] = 1.0
+                Wk, J[:, k], brk = orc.grp(WL, WR, e[:4], e[4:], G)
+                assert brk == key[0]
+                np.testing.assert_array_equal(Wk, W0)
+            S = np.concatenate([SL, SR])
+            a, b = rng.uniform(-3, 3, size=2)
+            S2 = rng.normal(size=8)
+            _, Wt1, _ = orc.grp(WL, WR, SL, SR, G)
+            _, Wt2, _ = orc.grp(WL, WR, S2[:4], S2[4:], G)
+            _, Wt12, _ = orc.grp(WL, WR, a * SL + b * S2[:4], a * SR + b * S2[4:], G)
+            scale = np.abs(J).sum(axis=1) * (abs(a) + abs(b)) * 4 + 1e-300
+            assert np.all(np.abs(Wt12 - (a * Wt1 + b * Wt2)) <= 1e-13 * scale)
+            assert np.all(np.abs(Wt1 - J @ S) <= 1e-13 * (np.abs(J) @ np.abs(S) + 1e-300))
+
+
+@pytest.mark.parametrize("exact_pow2", [True, False])
+def test_grp_dimensional_scaling(orc, exact_pow2):
+    """(c) Scaling laws of the Euler equations that the relaxation GRP keeps:
+    (rho, p) -> lam (rho, p) with (rho_x, p_x) scaled alike leaves u, v, e
+    and their derivatives unchanged and scales rho_t, p_t by lam;
+    (u, v, p) -> (kap u, kap v, kap^2 p) with slopes scaled alike scales
+    (rho, p, u, v, e)_t by (kap, kap^3, kap^2, kap^2, kap^3).  With powers of
+    two every operation commutes with the scaling, so the results are equal
+    bit for bit."""
+    rng = np.random.default_rng(103 + exact_pow2)
+    for key, cases in _grp_bins(orc, rng, 20).items():
+        for WL, WR, SL, SR in cases:
+            W, Wt, br = orc.grp(WL, WR, SL, SR, G)
+            lam = 2.0 ** rng.integers(-3, 4) if exact_pow2 else rng.uniform(0.3, 3.0)
+            kap = 2.0 ** rng.integers(-3, 4) if exact_pow2 else rng.uniform(0.3, 3.0)
+            m = np.array([lam, lam, 1, 1])
+            Wl, Wtl, brl = orc.grp(WL * m, WR * m, SL * m, SR * m, G)
+            mv = np.array([1, kap * kap, kap, kap])
+            Wk, Wtk, brk = orc.grp(WL * mv, WR * mv, SL * mv, SR * mv, G)
+            assert brl == br and brk == br
+            expW_l, expWt_l = W * np.array([lam, lam, 1, 1, 1]), Wt * np.array([lam, lam, 1, 1, 1])
+            expW_k = W * np.array([1, kap ** 2, kap, kap, kap ** 2])
+            expWt_k = Wt * np.array([kap, kap ** 3, kap ** 2, kap ** 2, kap ** 3])
+            if exact_pow2:
+                np.testing.assert_array_equal(Wl, expW_l)
+                np.testing.assert_array_equal(Wtl, expWt_l)
+                np.testing.assert_array_equal(Wk, expW_k)
+                np.testing.assert_array_equal(Wtk, expWt_k)
+            else:
+                for got, exp in ((Wl, expW_l), (Wtl, expWt_l), (Wk, expW_k), (Wtk, expWt_k)):
+                    np.testing.assert_allclose(got, exp, rtol=1e-11,
+                                               atol=1e-12 * np.abs(exp).max())
+
+
+def _acoustic_interface(WL, WR, SL, SR):
+    """Linear acoustics at a material interface at rest (u = 0 on both sides,
+    equal pressure): the right-going invariant p + Z_l u arrives from the
+    left and the left-going p - Z_r u from the right (Z = rho a), so with
+    k_l = -(p_x + Z u_x)_l and k_r = -(p_x - Z u_x)_r
+    p_t = (Z_r a_l k_l - Z_l a_r k_r)/(Z_l + Z_r),
+    u_t = (a_l k_l + a_r k_r)/(Z_l + Z_r); the interface is a stationary
+    particle path on the right side (u* = 0 selects it), where entropy is
+    constant in time: rho_t = p_t / a_r^2 and e_t = p_t / (gamma rho_r)."""
+    rl, p, _, _ = WL
+    rr, _, _, vr = WR
+    al, ar = math.sqrt(G * p / rl), math.sqrt(G * p / rr)
+    Zl, Zr = rl * al, rr * ar
+    kl = -(SL[1] + Zl * SL[2])
+    kr = -(SR[1] - Zr * SR[2])
+    pt = (Zr * al * kl - Zl * ar * kr) / (Zl + Zr)
+    ut = (al * kl + ar * kr) / (Zl + Zr)
+    return (np.array([rr, p, 0.0, vr, p / rr / (G - 1)]),
+            np.array([pt / ar ** 2, pt, ut, 0.0, pt / (G * rr)]))
+
+
+@pytest.mark.parametrize("mismatch", [False, True])
+def test_grp_acoustic_limit(orc, mismatch):
+    """(d) u_L = u_R = 0, equal pressures, slopes S_L != S_R: the GRP gives
+    the characteristic-upwind solution of linear acoustics.  Without
+    mismatch W_L = W_R (p_t = a (k_l - k_r)/2, u_t = (k_l + k_r)/(2 rho));
+    with mismatch the densities differ, so the two sides have different
+    impedances and every left/right operand of the star formulas is
+    distinguishable."""
+    rng = np.random.default_rng(104 + mismatch)
+    for _ in range(300):
+        rl, p = rng.uniform(0.2, 3), rng.uniform(0.2, 3)
+        rr = rng.uniform(0.2, 3) if mismatch else rl
+        WL = np.array([rl, p, 0.0, rng.normal()])
+        WR = np.array([rr, p, 0.0, rng.normal() if mismatch else WL[3]])
+        SL, SR = rng.normal(size=4), rng.normal(size=4)
+        W, Wt, br = orc.grp(WL, WR, SL, SR, G)
+        assert br == 2  # u* = 0 exactly: the right side of the contact
+        expW, expWt = _acoustic_interface(WL, WR, SL, SR)
+        np.testing.assert_allclose(W, expW, rtol=1e-14)
+        np.testing.assert_allclose(Wt, expWt, rtol=1e-11, atol=1e-12 * np.abs(Wt).max())
+
+
+def _relaxation_interface(WL, WR, SL, SR, cl, cr):
+    """Interface values and Eulerian time derivatives at x = 0 of the
+    Lagrangian relaxation system (Suliciu; Bouchut 2004) with the impedance
+    c frozen on each side:  tau_t - u_m = 0, u_t + pi_m = 0,
+    pi_t + c^2 u_m = 0 (m: mass coordinate, dm = rho dx), v_t = 0 and
+    (e - pi^2 / 2c^2)_t = 0, for piecewise-linear data.  Solved by
+    characteristics from the numerical eigen-decomposition of each side's
+    3 x 3 system:
+      * supersonic: the upwind data itself, q_t = -A q_m - rho u q_m;
+      * star region: the invariants arriving from the data fix (u*, pi*)
+        and their material derivatives at the contact, the zero-speed field
+        fixes tau* on the contact's side K, the field reflected into side K
+        is carried from the contact, and the Eulerian derivative at the
+        fixed interface is D/Dt - rho* u* d/dm on side K (the contact moves
+        with u*, K = left if u* > 0)."""
+    def side(W, S, c):
+        r, p, u, v = W
+        rx, px, ux, vx = S
+        z = np.array([1.0 / r, u, p])
+        zm = np.array([-rx / r ** 2, ux, px]) / r  # d/dm = (1/rho) d/dx
+        A = np.array([[0.0, -1.0, 0.0], [0.0, 0.0, 1.0], [0.0, c * c, 0.0]])
+        lam, Rv = np.linalg.eig(A)
+        order = np.argsort(lam.real)  # -c, 0, +c
+        lam, Rv = lam.real[order], Rv.real[:, order]
+        L = np.linalg.inv(Rv)
+        ex = (px / r - p * rx / r ** 2) / (G - 1)
+        return dict(r=r, p=p, u=u, v=v, c=c, z=z, zm=zm, A=A, lam=lam, R=Rv, L=L,
+                    e=p / r / (G - 1), em=ex / r, vm=vx / r)
+
+    sl, sr = side(WL, SL, cl), side(WR, SR, cr)
+    if WL[2] - cl / WL[0] >= 0.0 or WR[2] + cr / WR[0] <= 0.0:
+        K = sl if WL[2] - cl / WL[0] >= 0.0 else sr
+        r, u, p = K["r"], K["u"], K["p"]
+        Dz = -K["A"] @ K["zm"]
+        zt = Dz - r * u * K["zm"]
+        De = -p * K["zm"][1]           # e_t = -pi u_m
+        et = De - r * u * K["em"]
+        W = np.array([r, p, u, K["v"], K["e"]])
+        return W, np.array([-r * r * zt[0], zt[2], zt[1], -r * u * K["vm"], et])
+    # incoming invariants: +c from the left data, -c from the right data;
+    # neither involves tau (checked), so they fix (u*, pi*) and D/Dt of both
+    lp, lm = sl["L"][2], sr["L"][0]
+    assert abs(lp[0]) < 1e-12 * np.abs(lp).max() and abs(lm[0]) < 1e-12 * np.abs(lm).max()
+    M = np.array([lp[1:], lm[1:]])
+    us, ps = np.linalg.solve(M, [lp @ sl["z"], lm @ sr["z"]])
+    Du, Dp = np.linalg.solve(M, [-sl["lam"][2] * (lp @ sl["zm"]), -sr["lam"][0] * (lm @ sr["zm"])])
+    K = sl if us > 0.0 else sr
+    l0 = K["L"][1]                    # zero-speed field: stationary on each side
+    taus = (l0 @ K["z"] - l0[1] * us - l0[2] * ps) / l0[0]
+    Dtau = -(l0[1] * Du + l0[2] * Dp) / l0[0]
+    Dz = np.array([Dtau, Du, Dp])
+    # d/dm of the characteristic fields in the star region on side K
+    wm = K["L"] @ K["zm"]             # fields arriving from the data
+    refl = 2 if K is sr else 0        # field carried from the contact into K
+    wm[refl] = -(K["L"][refl] @ Dz) / K["lam"][refl]
+    zm = K["R"] @ wm
+    rs = 1.0 / taus
+    zt = Dz - rs * us * zm
+    c2 = K["c"] ** 2
+    es = K["e"] + (ps * ps - K["p"] ** 2) / (2 * c2)
+    em = (K["em"] - K["p"] * K["zm"][2] / c2) + ps * zm[2] / c2
+    et = ps / c2 * Dp - rs * us * em
+    W = np.array([rs, ps, us, K["v"], es])
+    return W, np.array([-rs * rs * zt[0], zt[2], zt[1], -rs * us * K["vm"], et])
+
+
+def test_grp_matches_relaxation_system_with_frozen_impedance(orc):
+    """Every branch, discontinuous data, u* of either sign, both pressure
+    orderings, relaxation max-terms on and off: where the slopes leave the
+    Lagrangian sound speed unchanged (p rho_x + rho p_x = 0, i.e. c_x = 0 in
+    src/basicFuns.cuh:104-105) the GRP's values and time derivatives are
+    those of the relaxation system solved by characteristics above.  This
+    fixes every star-region term of src/basicFuns.cuh:147-185 except the
+    ones proportional to c_x; left/right operand swaps, dropped terms and
+    wrong powers of c or rho all fail it."""
+    rng = np.random.default_rng(105)
+
+    def frozen(rng, W):
+        s = rng.normal(size=4)
+        s[0] = -W[0] * s[1] / W[1]
+        return s
+
+    nmax = 0
+    for key, cases in _grp_bins(orc, rng, 60, slope_fn=frozen).items():
+        for WL, WR, SL, SR in cases:
+            W, Wt, br = orc.grp(WL, WR, SL, SR, G)
+            cl, cr = relax_speeds(WL, WR)
+            kap = (G + 1) / 2
+            active = (WR[1] - WL[1]) / (WR[0] * math.sqrt(G * WR[1] / WR[0])) + WL[2] - WR[2] > 0
+            nmax += bool(active)
+            expW, expWt = _relaxation_interface(WL, WR, SL, SR, cl, cr)
+            np.testing.assert_allclose(W, expW, rtol=1e-11, atol=1e-13)
+            np.testing.assert_allclose(Wt, expWt, rtol=1e-9, atol=1e-10 * np.abs(expWt).max())
+    assert nmax > 50  # the relaxation max-term was active in many cases
