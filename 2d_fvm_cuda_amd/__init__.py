@@ -23,6 +23,7 @@ from . import problems  # noqa: F401  (re-exported problem library)
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libfvm2d.so")
+PROBES_LIB_PATH = os.path.join(PKG, "libfvm2d_probes.so")
 
 FVM2D_OK = 0
 
@@ -67,7 +68,7 @@ EXPORTS = [
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
     "fvm2d_batch_status", "fvm2d_batch_download", "fvm2d_batch_set_launch",
-    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_strip_layout", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe", "fvm2d_recip_probe",
+    "fvm2d_batch_autotune", "fvm2d_batch_destroy", "fvm2d_strip_layout",
     "fvm2d_adj_create", "fvm2d_adj_forward", "fvm2d_adj_backward", "fvm2d_adj_get_stats",
     "fvm2d_adj_set_launch", "fvm2d_adj_destroy",
 ]
@@ -121,9 +122,6 @@ def lib():
         "fvm2d_strip_layout": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_int),
                                                                  ctypes.c_int,
                                                                  ctypes.POINTER(ctypes.c_int)]),
-        "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
-        "fvm2d_valu_probe": (ctypes.c_int, [ctypes.c_int, _dp]),
-        "fvm2d_recip_probe": (ctypes.c_int, [_dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -146,11 +144,44 @@ def check(rc: int, what: str = "") -> None:
         raise FVM2DError(f"{what} failed ({rc}): {msg}")
 
 
+PROBE_EXPORTS = ["fvm2d_probe_last_error", "fvm2d_hbm_copy_probe", "fvm2d_valu_probe",
+                 "fvm2d_recip_probe"]
+_probes = None
+
+
+def probes_lib():
+    """Load libfvm2d_probes.so, the measurement probes (include/fvm2d_probes.h)."""
+    global _probes
+    if _probes is not None:
+        return _probes
+    if not os.path.exists(PROBES_LIB_PATH):
+        raise FVM2DError(f"{PROBES_LIB_PATH} is not built (run 2d_fvm_cuda_amd/build.py)")
+    L = ctypes.CDLL(PROBES_LIB_PATH)
+    sig = {
+        "fvm2d_probe_last_error": (ctypes.c_char_p, []),
+        "fvm2d_hbm_copy_probe": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp]),
+        "fvm2d_valu_probe": (ctypes.c_int, [ctypes.c_int, _dp]),
+        "fvm2d_recip_probe": (ctypes.c_int, [_dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _probes = L
+    return L
+
+
+def _probe_check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = probes_lib().fvm2d_probe_last_error().decode(errors="replace")
+        raise FVM2DError(f"{what} failed ({rc}): {msg}")
+
+
 def valu_probe(kind: int) -> float:
     """Wave-level instructions per SIMD per second of one VALU kind (0 fma_f64,
     1 mul_f64, 2 add_f64, 3 rcp_f64, 4 32-bit integer ops)."""
     r = ctypes.c_double(0.0)
-    check(lib().fvm2d_valu_probe(int(kind), ctypes.byref(r)), "fvm2d_valu_probe")
+    _probe_check(probes_lib().fvm2d_valu_probe(int(kind), ctypes.byref(r)), "fvm2d_valu_probe")
     return r.value
 
 
@@ -158,7 +189,7 @@ def recip_probe():
     """Max relative errors (raw rcp, refined rcp, raw rsq, refined rsq) of
     the device reciprocals the FAST path uses."""
     out = (ctypes.c_double * 4)()
-    check(lib().fvm2d_recip_probe(out), "fvm2d_recip_probe")
+    _probe_check(probes_lib().fvm2d_recip_probe(out), "fvm2d_recip_probe")
     return tuple(out)
 
 
@@ -185,7 +216,8 @@ def hbm_copy_probe(n_doubles: int, reps: int) -> float:
     """Mean ms of one plain device copy of n_doubles FP64 values (the HBM
     counter calibration kernel)."""
     ms = ctypes.c_double(0.0)
-    check(lib().fvm2d_hbm_copy_probe(int(n_doubles), int(reps), ctypes.byref(ms)), "fvm2d_hbm_copy_probe")
+    _probe_check(probes_lib().fvm2d_hbm_copy_probe(int(n_doubles), int(reps), ctypes.byref(ms)),
+                 "fvm2d_hbm_copy_probe")
     return ms.value
 
 

@@ -195,8 +195,28 @@ struct Engine {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         cu_count = prop.multiProcessorCount;
+        if (n == 1) return set_launch(one_mesh_rows(), 4, 0);
         R = default_strip_rows();
-        return set_launch(R, 4, n > 1 ? 1 : 0);
+        return set_launch(R, 4, 1);
+    }
+
+    // Strip height of one mesh launched without tuning (four waves per
+    // block, one wave per task): the lowest strips whose count over all
+    // column windows still fits the fewest rounds of resident waves that
+    // keep strips at most 96 rows high.  mesh_strips then cuts them so that
+    // one or two rounds are filled exactly.  At 2000 x 2000 on 256 CUs this
+    // is one round at R = 34, the geometry the autotuner picks.
+    int one_mesh_rows() const {
+        const InstDesc &d = hd[0];
+        const long long H = d.ny - 4;
+        const long long slots = (long long)std::max(1, step_kernel_occupancy(exact, 4)) * 4 * cu_count;
+        for (long long k = 1; k <= 64; k++) {
+            const long long per_window = k * slots / d.nw;
+            if (per_window < 1) continue;
+            const long long r = (H + per_window - 1) / per_window;
+            if (r <= 96) return (int)std::max<long long>(4, r);
+        }
+        return 64;
     }
 
     int default_strip_rows() const {
@@ -279,34 +299,33 @@ struct Engine {
         if (nt > S && nt <= 2 * S) std::reverse(tasks.begin() + S, tasks.end());
     }
 
+    // Change the launch geometry.  On any failure the current geometry (task
+    // list, descriptors, grid and graph) is left as it was.
     int set_launch(int strip_rows, int waves_per_block, int persist) {
         const int newR = strip_rows > 0 ? strip_rows : R;
+        const int newW = waves_per_block > 0 ? std::min(4, waves_per_block) : wpb;
+        const int newP = persist >= 0 ? (persist ? 1 : 0) : persistent;
         {
             // task positions are ints: refuse geometries with more tasks
-            // (the round-filling cut at most doubles a mesh's count); the
-            // current geometry stays as it was
+            // (the round-filling cut at most doubles a mesh's count)
             long long total = 0;
             for (int i = 0; i < n; i++)
                 total += 2LL * hd[i].nw * ((hd[i].ny - 4 + newR - 1) / newR) + 1;
             if (total > (1LL << 30)) return fail(FVM2D_ERR_ARG, "strip rows too low for this mesh");
         }
-        R = newR;
-        if (waves_per_block > 0) wpb = std::min(4, waves_per_block);
-        if (persist >= 0) persistent = persist ? 1 : 0;
-        // host copy kept as a member: the upload below is asynchronous
-        std::vector<int4> &tasks = h_tasks;
-        tasks.clear();
+        std::vector<int4> tasks;
+        std::vector<InstDesc> nd = hd;
         // resident waves of the launch: blocks per CU x waves per block x CUs
-        const long long slots =
-            (long long)std::max(1, step_kernel_occupancy(exact, wpb)) * wpb * cu_count;
+        const int occ = std::max(1, step_kernel_occupancy(exact, newW));
+        const long long slots = (long long)occ * newW * cu_count;
         for (int i = 0; i < n; i++) {
-            InstDesc &d = hd[i];
-            d.R = R;
+            InstDesc &d = nd[i];
+            d.R = newR;
             d.task_base = (int)tasks.size();
-            mesh_strips(i, d.nw, d.ny, R, n == 1 && !persistent, slots, 4LL * cu_count, wpb, tasks);
+            mesh_strips(i, d.nw, d.ny, newR, n == 1 && !newP, slots, 4LL * cu_count, newW, tasks);
             d.ntask = (int)tasks.size() - d.task_base;
         }
-        ntasks = (int)tasks.size();
+        const int nt = (int)tasks.size();
         // Task order of one mesh whose tasks fill one round of two waves per
         // SIMD: the dispatcher gives the first 4 * cu_count positions one
         // SIMD each and puts position S + k on the SIMD of about position k
@@ -319,23 +338,36 @@ struct Engine {
         // Reversing the second round pairs each task with its mirror image
         // through the mesh centre instead (2.5% less time per 2000^2 step).
         // The order decides only which wave computes which task.
-        if (n == 1 && !persistent) mirror_second_round(tasks, 4LL * cu_count);
-        if (d_tasks) HIPCHK(hipFree(d_tasks));
-        d_tasks = nullptr;
-        HIPCHK(hipMalloc(&d_tasks, std::max(1, ntasks) * sizeof(int4)));
-        HIPCHK(hipMemcpyAsync(d_tasks, tasks.data(), ntasks * sizeof(int4), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, stream));
-        const int blocks_needed = (ntasks + wpb - 1) / wpb;
-        if (persistent) {
-            int occ = step_kernel_occupancy(exact, wpb);
-            if (occ < 1) occ = 1;
-            grid = std::min(blocks_needed, occ * cu_count);
-        } else {
-            grid = blocks_needed;
-        }
-        if (grid < 1) grid = 1;
-        drop_graph();
+        if (n == 1 && !newP) mirror_second_round(tasks, 4LL * cu_count);
+        const int blocks_needed = (nt + newW - 1) / newW;
+        const int newgrid = std::max(1, newP ? std::min(blocks_needed, occ * cu_count) : blocks_needed);
+        // Upload into a new buffer; the old task list and descriptors stay
+        // valid until every step of this stream that uses them has run.
         HIPCHK(hipStreamSynchronize(stream));
+        int4 *dt_new = nullptr;
+        HIPCHK(hipMalloc(&dt_new, std::max(1, nt) * sizeof(int4)));
+        hipError_t e = hipMemcpyAsync(dt_new, tasks.data(), nt * sizeof(int4), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_desc, nd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, stream);
+        const hipError_t es = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = es;
+        if (e != hipSuccess) {
+            hipFree(dt_new);
+            // the descriptor copy may have landed partly: restore the current ones
+            hipMemcpy(d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice);
+            return fail(FVM2D_ERR_HIP, std::string("uploading the launch layout: ") + hipGetErrorString(e));
+        }
+        // the captured graph replays the old task list and grid
+        drop_graph();
+        if (d_tasks) hipFree(d_tasks);
+        d_tasks = dt_new;
+        hd.swap(nd);
+        h_tasks.swap(tasks);
+        ntasks = nt;
+        R = newR;
+        wpb = newW;
+        persistent = newP;
+        grid = newgrid;
         return FVM2D_OK;
     }
 
@@ -481,12 +513,16 @@ struct Engine {
 
     static constexpr int kGraphSteps = 16;
 
-    // Enqueue nsteps step launches (graph replays + a remainder of plain launches).
+    // Enqueue nsteps step launches: graph replays of kGraphSteps launches
+    // (the graph is captured only when at least one replay follows) and a
+    // remainder of plain launches.
     int enqueue(int nsteps) {
-        int rc = ensure_graph(kGraphSteps);
-        if (rc) return rc;
         int k = 0;
-        for (; k + kGraphSteps <= nsteps; k += kGraphSteps) HIPCHK(hipGraphLaunch(gexec, stream));
+        if (nsteps >= kGraphSteps) {
+            int rc = ensure_graph(kGraphSteps);
+            if (rc) return rc;
+            for (; k + kGraphSteps <= nsteps; k += kGraphSteps) HIPCHK(hipGraphLaunch(gexec, stream));
+        }
         for (; k < nsteps; k++) launch_one();
         HIPCHK(hipGetLastError());
         return FVM2D_OK;
@@ -521,16 +557,21 @@ struct Engine {
         return (int)std::min<long long>(best, 1 << 20);
     }
 
-    // Device-resident time loop until every instance is done.
+    // Device-resident time loop until every instance is done.  The timed
+    // region (ms_out) starts after the initial s_max and dt are known: the
+    // first chunk is sized from them (exactly, under a step limit), and a
+    // launch that finds its instance done exits at once.
     int run_to_completion(double *ms_out) {
         if (!begun) {
             int rc = begin();
             if (rc) return rc;
         }
-        HIPCHK(hipEventRecord(e0, stream));
         int rc = read_ctl();
         if (rc) return rc;
-        int chunk = 8;
+        int chunk = std::max(1, remaining_estimate());
+        // capture the step graph before the timed region when the loop uses it
+        if (chunk >= kGraphSteps && (rc = ensure_graph(kGraphSteps))) return rc;
+        HIPCHK(hipEventRecord(e0, stream));
         while (!all_done()) {
             rc = enqueue(chunk);
             if (rc) return rc;

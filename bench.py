@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+CU_COUNT = 256               # MI355X compute units (4 SIMDs each)
+# FP64 vector peak 78.6 TFLOP/s = 256 CUs x 4 SIMDs x 16 lanes x 2 flops x 2.4 GHz:
+# one wave64 v_fma_f64 per SIMD every 4 clocks
+FP64_ISSUE_PEAK = 6.0e8      # wave-level FP64 VALU instructions per SIMD per second
 BYTES_PER_CELL_UPDATE = 288  # 4 sweeps x (32 B read + 32 B write) + 32 B partner read
 COPY_PROBE_BYTES = 1 << 28   # 256 MiB read + 256 MiB written per copy launch
 
@@ -40,14 +44,17 @@ def cpu_baseline(U0, steps: int, threads: int):
     return cells * n / (ms / 1000.0), n, ms
 
 
-def traffic_record():
+def counter_record():
+    """Per-launch hardware counters of the step kernel at 2000^2 (HBM bytes
+    from FETCH_SIZE / WRITE_SIZE scaled by the copy-probe calibration, VALU
+    instructions from SQ_INSTS_VALU), committed under profiles/."""
     path = os.path.join(ROOT, "profiles", "step_kernel_2000_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
-        return json.load(open(path)).get("bytes_per_launch")
+        return json.load(open(path))
     except Exception:
-        return None
+        return {}
 
 
 def main():
@@ -64,7 +71,7 @@ def main():
                     help="strip_rows,waves_per_block,persistent: fixed geometry instead of autotuning")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
-    ap.add_argument("--leg", choices=["batched", "adjoint", "batched_adjoint", "peak_8192"],
+    ap.add_argument("--leg", choices=["dropin", "batched", "adjoint", "batched_adjoint", "peak_8192"],
                     help="run only this extra leg and print its record")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the VALU throughput probe (instructions per SIMD per second)")
@@ -97,7 +104,8 @@ def main():
 
     if args.leg:
         extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
-        fn = {"batched": extras.batched, "adjoint": extras.adjoint,
+        fn = {"dropin": lambda f: extras.dropin(f, args.n, args.steps or -1, fvm.MODE_FAST),
+              "batched": extras.batched, "adjoint": extras.adjoint,
               "batched_adjoint": extras.batched_adjoint, "peak_8192": extras.peak}[args.leg]
         print(json.dumps({args.leg: fn(fvm)}))
         return
@@ -125,20 +133,40 @@ def main():
         s.upload(U0)
 
     steps, t_final, ms = s.run()
+    s.close()
     cells = N * N
     value = cells * steps / (ms / 1000.0)
-
     # The timed region launches the step kernel once per step (graph replays
-    # on the solver's stream, bracketed by HIP events), so its average
-    # launch duration is ms / steps; back-to-back and single-launch timings
-    # of a fresh copy of the workload are reported beside it.
+    # on the solver's stream, bracketed by HIP events), so its mean launch
+    # duration is ms / steps.  Beside it: back-to-back launches on a fresh
+    # copy of the workload with no step limit (every launch does a full step).
     k_avg_ms = ms / max(steps, 1)
-    s.upload(U0)
-    s.begin()
-    k_b2b_ms, k_min_ms = s.time_kernel(50)
-    s.close()
+    kt = fvm.Solver(N, N, **dict(prm, t_end=1e30, max_steps=-1))
+    kt.set_launch(*launch)
+    kt.upload(U0)
+    kt.begin()
+    kt.advance(5)
+    kt.sync()
+    k_b2b_ms, k_min_ms = kt.time_kernel(50)
+    kt.close()
 
     achieved = BYTES_PER_CELL_UPDATE * cells / (k_avg_ms / 1000.0) / 1e9
+    ctr = counter_record() if (N == 2000 and args.mode == "fast") else {}
+    traffic = ctr.get("bytes_per_launch")
+    valu_insts = ctr.get("valu_insts_per_launch")
+    roof = {"bound": "fp64-valu", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "fvm_step_kernel", "kernel_avg_ms": k_avg_ms,
+            "kernel_back_to_back_ms": k_b2b_ms, "kernel_min_ms": k_min_ms,
+            "bytes_per_cell_update": BYTES_PER_CELL_UPDATE}
+    if traffic:
+        gbs = traffic / (k_avg_ms / 1000.0) / 1e9
+        roof["hbm_measured"] = {"bytes_per_launch": traffic, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    if valu_insts:
+        per_simd = valu_insts / (k_avg_ms / 1000.0) / (4 * CU_COUNT)
+        roof["valu"] = {"insts_per_launch": valu_insts, "per_simd_per_s": per_simd,
+                        "fp64_issue_peak_per_simd_per_s": FP64_ISSUE_PEAK,
+                        "issue_frac": per_simd / FP64_ISSUE_PEAK}
     out = {
         "metric": "FP64 cell-updates/s at 2000^2 (% HBM roofline); batched and adjoint vs plain",
         "value": value,
@@ -157,19 +185,17 @@ def main():
                    "mesh": [N, N], "t_end": 0.3, "t_final": t_final, "mode": args.mode,
                    "launch": {"strip_rows": launch[0], "waves_per_block": launch[1],
                               "persistent": launch[2]}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_record(),
-                     "kernel": "fvm_step_kernel", "kernel_avg_ms": k_avg_ms,
-                     "kernel_back_to_back_ms": k_b2b_ms, "kernel_min_ms": k_min_ms,
-                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE},
+        "roofline": roof,
     }
     if tuned_ms is not None:
         out["config"]["autotune_best_ms_per_step"] = tuned_ms
 
     if not args.no_extras:
         extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
-        for key, fn in (("batched", extras.batched), ("adjoint", extras.adjoint),
-                        ("batched_adjoint", extras.batched_adjoint), ("peak_8192", extras.peak)):
+        legs = (("dropin", lambda f: extras.dropin(f, N, steps, mode, out["ms_per_step"])),
+                ("batched", extras.batched), ("adjoint", extras.adjoint),
+                ("batched_adjoint", extras.batched_adjoint), ("peak_8192", extras.peak))
+        for key, fn in legs:
             try:
                 out[key] = fn(fvm)
             except Exception as e:  # a failing leg is reported, not fatal to the headline

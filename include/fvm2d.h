@@ -147,27 +147,6 @@ int fvm2d_batch_destroy(fvm2d_batch *b);
 int fvm2d_strip_layout(int nx, int ny, int strip_rows, int waves_per_block, int persistent,
                        int cu_count, int blocks_per_cu, int *out, int max_tasks, int *ntasks_out);
 
-/* Diagnostic with no counterpart in the reference: reps launches of a plain
- * device-to-device copy of n_doubles FP64 values (16-byte accesses, two
- * buffers allocated and freed by the call).  Its FETCH_SIZE / WRITE_SIZE
- * counters calibrate the step kernel's measured HBM traffic.  avg_ms_out:
- * mean duration of one copy. */
-int fvm2d_hbm_copy_probe(long long n_doubles, int reps, double *avg_ms_out);
-
-/* Diagnostic with no counterpart in the reference: sustained wave-level
- * instruction rate per SIMD of one VALU kind with 16 independent chains per
- * lane on every SIMD (kind 0 v_fma_f64, 1 v_mul_f64, 2 v_add_f64,
- * 3 v_rcp_f64, 4 32-bit integer ops).  Bounds what an FP64-issue-bound
- * kernel can reach on the card at its sustained clock. */
-int fvm2d_valu_probe(int kind, double *per_simd_per_s);
-
-/* Diagnostic with no counterpart in the reference: maximum relative error,
- * over 2^18 arguments 2^u (u uniform in [-8, 8)), of the device's raw
- * v_rcp_f64, the FAST path's refined reciprocal, raw v_rsq_f64 and the
- * refined reciprocal square root (out[0..3]), against long-double host
- * references. */
-int fvm2d_recip_probe(double *max_rel_err4);
-
 /* ---- adjoint (no counterpart in the reference) ------------------------- *
  * Hand-written reverse mode of the full split step (limiter and GRP
  * differentiated along the branch taken, dt differentiated through its

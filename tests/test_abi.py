@@ -11,10 +11,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(header=None):
     names = set()
     for h in os.listdir(os.path.join(ROOT, "include")):
-        if not h.endswith(".h"):
+        if not h.endswith(".h") or (header is not None and h != header):
             continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -24,13 +24,34 @@ def declared_symbols():
 
 def test_library_exports_every_declared_symbol(fvm):
     lib = fvm.lib()
-    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    missing = [n for n in declared_symbols("fvm2d.h") if not hasattr(lib, n)]
     assert not missing, missing
-    assert len(declared_symbols()) >= 25
+    assert len(declared_symbols("fvm2d.h")) >= 25
+
+
+def test_probes_library_exports_every_declared_symbol(fvm):
+    lib = fvm.probes_lib()
+    declared = declared_symbols("fvm2d_probes.h")
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(fvm.PROBE_EXPORTS) == set(declared)
+
+
+def test_solver_library_holds_no_probes(fvm):
+    """The measurement probes live in libfvm2d_probes.so only."""
+    lib = fvm.lib()
+    for n in declared_symbols("fvm2d_probes.h"):
+        assert not hasattr(lib, n), n
+
+
+def test_every_header_is_covered():
+    """Each public header's declarations belong to one of the two libraries."""
+    hs = sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h"))
+    assert hs == ["fvm2d.h", "fvm2d_probes.h"]
 
 
 def test_python_binding_covers_exports(fvm):
-    assert set(fvm.EXPORTS) <= set(declared_symbols())
+    assert set(fvm.EXPORTS) <= set(declared_symbols("fvm2d.h"))
 
 
 def test_params_default_matches_reference(fvm):
