@@ -63,7 +63,8 @@ class Params(ctypes.Structure):
 EXPORTS = [
     "fvm2d_params_default", "fvm2d_last_error", "fvm2d_device_available", "fvm2d_solve",
     "fvm2d_create", "fvm2d_set_params", "fvm2d_upload", "fvm2d_upload_planes_device",
-    "fvm2d_download_planes_device", "fvm2d_run", "fvm2d_download", "fvm2d_steps_total",
+    "fvm2d_download_planes_device", "fvm2d_run", "fvm2d_download", "fvm2d_download_frame",
+    "fvm2d_steps_total",
     "fvm2d_destroy", "fvm2d_set_launch", "fvm2d_get_launch", "fvm2d_autotune", "fvm2d_begin",
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
     "fvm2d_batch_create", "fvm2d_batch_upload", "fvm2d_batch_begin", "fvm2d_batch_run",
@@ -77,6 +78,23 @@ _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
+
+
+def hip_runtimes() -> list:
+    """Paths of the HIP runtime libraries mapped into this process.
+
+    PyTorch's ROCm wheel bundles its own HIP runtime under the soname of the
+    system's (libamdhip64.so.7) while its libraries ask for it by another
+    name.  Imported before libfvm2d is loaded, torch's copy is the only HIP
+    runtime in the process: libfvm2d binds to it by soname and device
+    pointers of torch tensors are valid for the library.  Imported after,
+    torch maps a second runtime beside the system's, and only host memory
+    can cross between the two."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({ln.split()[-1] for ln in f if "libamdhip64" in ln})
+    except OSError:
+        return []
 
 
 def lib():
@@ -100,6 +118,7 @@ def lib():
         "fvm2d_download_planes_device": (ctypes.c_int, [_vp, _vp]),
         "fvm2d_run": (ctypes.c_int, [_vp, _ip, _dp, _dp]),
         "fvm2d_download": (ctypes.c_int, [_vp, _dp]),
+        "fvm2d_download_frame": (ctypes.c_int, [_vp, _dp]),
         "fvm2d_steps_total": (ctypes.c_int, [_vp]),
         "fvm2d_destroy": (ctypes.c_int, [_vp]),
         "fvm2d_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -284,6 +303,13 @@ class Solver:
     def download(self) -> np.ndarray:
         U = np.empty((self.ny, self.nx, 5))
         check(lib().fvm2d_download(self.h, _ptr(U)), "fvm2d_download")
+        return U
+
+    def download_frame(self) -> np.ndarray:
+        """Inside a time loop: the state after the last step as the
+        reference's record mode copies it (before that step's ghost fill)."""
+        U = np.empty((self.ny, self.nx, 5))
+        check(lib().fvm2d_download_frame(self.h, _ptr(U)), "fvm2d_download_frame")
         return U
 
     def upload_planes_device(self, ptr: int):
@@ -478,9 +504,9 @@ class RGRP_2D_CUDA:
         """Record mode (src/wrappers.cuh:253-261): a frame of the density
         after the first step and after every gapPerFrame further steps,
         named by the index of the step (0, gap, 2 gap, ...).  The loop runs
-        on the device in chunks of gapPerFrame steps with a download per
-        frame; the frame holds the state after that step's ghost fill (the
-        reference copies it out just before the fill)."""
+        on the device in chunks of gapPerFrame steps; each frame is the
+        state as the reference copies it out, after the branch average and
+        before that step's ghost fill (fvm2d_download_frame)."""
         import time
         from .io import write_frame_pgm
         gap = max(1, self._gap)
@@ -498,7 +524,7 @@ class RGRP_2D_CUDA:
                 if steps < due:
                     break
                 done_steps = steps
-                write_frame_pgm(self._recordDir, s.download()[..., 0], due - 1)
+                write_frame_pgm(self._recordDir, s.download_frame()[..., 0], due - 1)
                 due += gap
             ms = (time.perf_counter() - t0) * 1e3
             U = s.download()

@@ -2,20 +2,21 @@
 ``torch.autograd.Function``.
 
     adj = Adjoint([(ny, nx)], [params], nsteps=400, checkpoint_slots=32)
-    U_N = FVMSolve.apply(adj, U0)          # U0: float64 CPU tensor [4, ny, nx]
+    U_N = FVMSolve.apply(adj, U0)          # U0: float64 tensor [4, ny, nx]
     loss = 0.5 * ((U_N[0] - target) ** 2).sum()
     loss.backward()                        # U0.grad = dL/dU0
 
 States are conserved planes (rho, rho u, rho v, E) in float64, row 0 =
-north, ghost layers included.  Tensors cross the boundary in host memory:
-the library stages them into HBM and runs the whole forward and reverse
-sweeps on the GPU.  (PyTorch's wheels bundle their own HIP runtime, which
-cannot initialise a device that libfvm2d's runtime already holds in the same
-process, so torch's GPU memory is not used here.)  The forward runs ``nsteps`` steps with
+north, ghost layers included.  CPU tensors (and numpy arrays) cross the
+boundary in host memory and the library stages them into HBM; CUDA tensors
+cross as device pointers (fvm2d_adj_forward / fvm2d_adj_backward) with no
+host copy.  That needs one HIP runtime in the process: import torch before
+this package loads libfvm2d, so that libfvm2d binds to torch's runtime
+(see ``hip_runtimes``).  The forward runs ``nsteps`` steps with
 dt = CFL h / s_max each (no t_end clamp) and stores binomial checkpoints; the
-backward recomputes from them and runs the reverse of every step
-(fvm2d_adj_backward).  Several instances (a batch) advance together:
-``FVMSolve.apply(adj, U0_a, U0_b, ...)`` returns one tensor per instance.
+backward recomputes from them and runs the reverse of every step.  Several
+instances (a batch) advance together: ``FVMSolve.apply(adj, U0_a, U0_b, ...)``
+returns one tensor per instance.
 """
 from __future__ import annotations
 
@@ -76,7 +77,8 @@ def _as_host(x):
     import numpy as np
     if hasattr(x, "detach"):
         if x.device.type != "cpu":
-            raise ValueError("pass CPU tensors: the library stages them into HBM itself")
+            raise ValueError("pass every state of a call on the same side: all CUDA tensors "
+                             "(device pointers) or all host arrays / CPU tensors")
         x = x.detach().numpy()
     return np.ascontiguousarray(x, dtype=np.float64)
 
@@ -128,12 +130,43 @@ class Adjoint:
         check(fn(self.h, _ptrs(ins), _ptrs(outs)), what)
         return outs
 
+    def _device_call(self, fn, xs, what):
+        """Device-pointer call on CUDA tensors; returns CUDA tensors."""
+        import torch
+        from . import check, hip_runtimes
+        if len(hip_runtimes()) != 1:
+            raise RuntimeError("CUDA tensors need a single HIP runtime in the process: import "
+                               "torch before libfvm2d is loaded")
+        assert len(xs) == self.n, "one state per instance"
+        ins = []
+        for x, (ny, nx) in zip(xs, self.shapes):
+            if tuple(x.shape) != (4, ny, nx):
+                raise ValueError(f"expected shape (4, {ny}, {nx}), got {tuple(x.shape)}")
+            ins.append(x.detach().to(torch.float64).contiguous())
+        outs = [torch.empty_like(a) for a in ins]
+        arr_in, arr_out = (_vp * self.n)(), (_vp * self.n)()
+        for i in range(self.n):
+            arr_in[i], arr_out[i] = ins[i].data_ptr(), outs[i].data_ptr()
+        torch.cuda.synchronize(ins[0].device)  # inputs written by torch's streams
+        check(fn(self.h, arr_in, arr_out), what)  # returns after its own stream is done
+        return outs
+
+    @staticmethod
+    def _on_device(xs):
+        return all(hasattr(x, "is_cuda") and x.is_cuda for x in xs)
+
     def forward(self, U0s):
-        """Conserved planes U_N (numpy) of every instance after nsteps steps."""
+        """Conserved planes U_N of every instance after nsteps steps: CUDA
+        tensors for CUDA inputs, numpy arrays otherwise."""
+        if self._on_device(U0s):
+            return self._device_call(self._lib.fvm2d_adj_forward, U0s, "fvm2d_adj_forward")
         return self._call(self._lib.fvm2d_adj_forward_host, U0s, "fvm2d_adj_forward_host")
 
     def backward(self, gNs):
-        """dL/dU_0 (numpy) of every instance given dL/dU_N, for the last forward."""
+        """dL/dU_0 of every instance given dL/dU_N, for the last forward
+        (CUDA tensors for CUDA inputs, numpy arrays otherwise)."""
+        if self._on_device(gNs):
+            return self._device_call(self._lib.fvm2d_adj_backward, gNs, "fvm2d_adj_backward")
         return self._call(self._lib.fvm2d_adj_backward_host, gNs, "fvm2d_adj_backward_host")
 
     def stats(self) -> dict:
@@ -158,13 +191,14 @@ def _function():
         @staticmethod
         def forward(ctx, adj, *U0s):
             ctx.adj = adj
-            outs = [torch.from_numpy(o) for o in adj.forward(list(U0s))]
+            outs = adj.forward(list(U0s))
+            outs = [o if torch.is_tensor(o) else torch.from_numpy(o) for o in outs]
             return outs[0] if len(outs) == 1 else tuple(outs)
 
         @staticmethod
         def backward(ctx, *gNs):
             gs = ctx.adj.backward([g.contiguous() for g in gNs])
-            return (None, *[torch.from_numpy(g) for g in gs])
+            return (None, *[g if torch.is_tensor(g) else torch.from_numpy(g) for g in gs])
 
     return FVMSolve
 

@@ -55,6 +55,7 @@ struct InstDesc {
 struct Ctl {
     double t;                      // current time
     double dt;                     // step size for the next step
+    double dt_used;                // step size of the last step taken
     double smax;                   // max |(u,v)| + c over the current state
     unsigned long long smax_next;  // bit pattern accumulator for the next state
     unsigned int waves_done;       // tasks of this instance finished in this launch

@@ -86,6 +86,7 @@ struct Engine {
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;                  // steps captured in gexec
     double *stage = nullptr;         // pinned staging for one instance
+    double *d_frame = nullptr;       // record-mode frame (4 planes of the largest mesh)
     long long stage_doubles = 0;
     std::vector<int> steps_total;
     bool begun = false;
@@ -108,6 +109,8 @@ struct Engine {
         d_t0 = nullptr;
         h_t0 = nullptr;
         if (stage) hipHostFree(stage);
+        if (d_frame) hipFree(d_frame);
+        d_frame = nullptr;
         if (e0) hipEventDestroy(e0);
         if (e1) hipEventDestroy(e1);
         if (stream) hipStreamDestroy(stream);
@@ -416,10 +419,15 @@ struct Engine {
 
     int download(int i, double *U5) {
         if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
-        const InstDesc &d = hd[i];
         const double *src = nullptr;
         int rc = current_buffer(i, &src);
         if (rc) return rc;
+        return copy_out(i, src, U5);
+    }
+
+    // Pitched 4-plane device state of instance i -> host vec5d [ny][nx][5].
+    int copy_out(int i, const double *src, double *U5) {
+        const InstDesc &d = hd[i];
         HIPCHK(hipMemcpyAsync(stage, src, 4 * d.plane * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         for (int j = 0; j < d.ny; j++) {
@@ -430,6 +438,30 @@ struct Engine {
             }
         }
         return FVM2D_OK;
+    }
+
+    // Record-mode frame of instance i inside a time loop: the state after
+    // the last completed step as it was before that step's ghost fill.
+    // The step's input is still in the other time-loop buffer (nothing has
+    // overwritten it), and dt_used is the step's own dt.
+    int frame(int i, double *U5) {
+        if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
+        if (!begun) return fail(FVM2D_ERR_STATE, "a frame needs a time loop (begin/advance)");
+        int rc = read_ctl();
+        if (rc) return rc;
+        const Ctl &c = h_ctl[i];
+        if (c.steps < 1) return fail(FVM2D_ERR_STATE, "no step taken yet in this time loop");
+        const InstDesc &d = hd[i];
+        if (!d_frame) {
+            long long mx = 0;
+            for (const auto &e : hd) mx = std::max(mx, e.plane);
+            HIPCHK(hipMalloc(&d_frame, 4 * mx * sizeof(double)));
+        }
+        const double *cur = (c.steps & 1) ? d.buf1 : d.buf0;
+        const double *prev = (c.steps & 1) ? d.buf0 : d.buf1;
+        launch_frame(exact, d, prev, cur, c.dt_used, d_frame, stream);
+        HIPCHK(hipGetLastError());
+        return copy_out(i, d_frame, U5);
     }
 
     int planes_device(int i, double *d_planes, bool in) {

@@ -15,6 +15,10 @@ void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int us
 void launch_ctl_start(const InstDesc *descs, Ctl *ctls, int ninst, const double *t0s,
                       hipStream_t stream);
 void launch_ctl_restore(Ctl *ctls, const Ctl *saved, int ninst, hipStream_t stream);
+// Record-mode frame of one mesh (the state before the last step's ghost
+// fill) from that step's input prev and output cur, into out (4 planes).
+void launch_frame(bool exact, const InstDesc &d, const double *prev, const double *cur, double dt,
+                  double *out, hipStream_t stream);
 int step_kernel_occupancy(bool exact, int wpb);
 
 }  // namespace fvm2d

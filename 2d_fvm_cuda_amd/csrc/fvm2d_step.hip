@@ -22,6 +22,8 @@
 // step counter on the device, so the host never waits inside the time loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fvm2d_device.h"
 #include "fvm2d_kernels.h"
 #include "fvm2d_ghost.h"
@@ -138,6 +140,7 @@ __device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_
         status = 1;
         done = 1;
     }
+    c->dt_used = c->dt;
     c->t = t_new;
     c->dt = dt;
     c->smax = smax;
@@ -347,6 +350,83 @@ __global__ void fvm_ctl_restore_kernel(Ctl *__restrict__ ctls, const Ctl *__rest
     c.smax_next = 0ull;
     c.waves_done = 0u;
     ctls[i] = c;
+}
+
+// ---------------------------------------------------------------- record frames
+// One cell of a 1-D sweep from the five cells of its line around it (line
+// order: m2, m1 are the "left" neighbours, p1, p2 the "right" ones); the same
+// operations as the lane-parallel sweeps.
+template <bool Exact>
+__device__ Q line_update(const Q &m2, const Q &m1, const Q &q0, const Q &p1, const Q &p2,
+                         const Line &L, const Gas &G, double dt) {
+    const Q s_m1 = sweep_slope<Exact>(m2, m1, q0, L);
+    const Q s_0 = sweep_slope<Exact>(m1, q0, p1, L);
+    const Q s_p1 = sweep_slope<Exact>(q0, p1, p2, L);
+    const double k = sweep_k<Exact>(dt, L);
+    IfState W, Wt;
+    grp<Exact>(sweep_face_plus<Exact>(m1, s_m1, L), sweep_face_minus<Exact>(q0, s_0, L), s_m1, s_0, G,
+               W, Wt);
+    const Q FL = flux<Exact>(W, Wt, k);
+    grp<Exact>(sweep_face_plus<Exact>(q0, s_0, L), sweep_face_minus<Exact>(p1, s_p1, L), s_0, s_p1, G,
+               W, Wt);
+    const Q FR = flux<Exact>(W, Wt, k);
+    return update<Exact>(q0, FL, FR, dt / L.h, G);
+}
+
+// The state after a step as the reference's record mode copies it out:
+// after the branch average, before the ghost fill (src/wrappers.cuh:248-261,
+// fill at :265-281).  From the step's input U (prev) and output (cur):
+//   interior cells          cur (the fill does not touch them);
+//   ghost rows, columns 2..nx-3: X(U) -- the X-sweep updates every row, the
+//       Y-sweep leaves rows 0, 1, ny-2, ny-1 alone, so both branches hold
+//       X(U) there and their average is X(U) exactly;
+//   ghost columns, rows 2..ny-3: Y(U), by the same argument;
+//   the four 2 x 2 corners: U (neither sweep touches them).
+// out: four planes in the state's padded layout.
+template <bool Exact>
+__global__ void __launch_bounds__(256) fvm_frame_kernel(InstDesc d, const double *__restrict__ prev,
+                                                        const double *__restrict__ cur, double dt,
+                                                        double *__restrict__ out) {
+    const long long n = (long long)d.nx * d.ny;
+    const long long P = d.plane;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        const bool grow = j < 2 || j > d.ny - 3, gcol = i < 2 || i > d.nx - 3;
+        auto at = [&](int jj, int ii) {
+            const long long q = (long long)jj * d.pitch + ii;
+            return Q{prev[q], prev[P + q], prev[2 * P + q], prev[3 * P + q]};
+        };
+        Q v;
+        if (!grow && !gcol) {
+            v = Q{cur[o], cur[P + o], cur[2 * P + o], cur[3 * P + o]};
+        } else if (grow && gcol) {
+            v = at(j, i);
+        } else if (grow) {
+            v = line_update<Exact>(at(j, i - 2), at(j, i - 1), at(j, i), at(j, i + 1), at(j, i + 2), d.lx,
+                                   d.gas, dt);
+        } else {
+            // Y line order: the left neighbour of row j is row j + 1, u and v swapped
+            v = swap_uv(line_update<Exact>(swap_uv(at(j + 2, i)), swap_uv(at(j + 1, i)),
+                                           swap_uv(at(j, i)), swap_uv(at(j - 1, i)),
+                                           swap_uv(at(j - 2, i)), d.ly, d.gas, dt));
+        }
+        out[o] = v.r;
+        out[P + o] = v.p;
+        out[2 * P + o] = v.un;
+        out[3 * P + o] = v.ut;
+    }
+}
+
+void launch_frame(bool exact, const InstDesc &d, const double *prev, const double *cur, double dt,
+                  double *out, hipStream_t stream) {
+    const long long n = (long long)d.nx * d.ny;
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+    if (exact)
+        hipLaunchKernelGGL(fvm_frame_kernel<true>, dim3(blocks), dim3(256), 0, stream, d, prev, cur, dt, out);
+    else
+        hipLaunchKernelGGL(fvm_frame_kernel<false>, dim3(blocks), dim3(256), 0, stream, d, prev, cur, dt, out);
 }
 
 // ---------------------------------------------------------------- launchers

@@ -89,6 +89,13 @@ int fvm2d_download_planes_device(fvm2d_solver *s, double *d_planes);
 int fvm2d_run(fvm2d_solver *s, int *steps_out, double *t_out, double *ms_out);
 /* solve() copies the final state out: src/wrappers.cuh:290-291. */
 int fvm2d_download(fvm2d_solver *s, double *U5);
+/* Record mode (src/wrappers.cuh:253-261): inside a time loop started by
+ * fvm2d_begin, after at least one step, the state after the last completed
+ * step as the reference copies it out for a frame -- after the branch
+ * average, before that step's ghost fill (the fill runs after the copy,
+ * src/wrappers.cuh:265-281).  Layout as fvm2d_download.  FVM2D_ERR_STATE
+ * outside a time loop or before its first step. */
+int fvm2d_download_frame(fvm2d_solver *s, double *U5);
 /* Total time steps over all runs (getTimeStagesCount, src/wrappers.cuh:313-316). */
 int fvm2d_steps_total(const fvm2d_solver *s);
 int fvm2d_destroy(fvm2d_solver *s);

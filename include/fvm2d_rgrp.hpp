@@ -12,7 +12,8 @@
 // writes a frame of the density after the first step and every gapPerFrame
 // steps, named by the step index, as an 8-bit binary PGM <index>.pgm (the
 // min-max normalised image the reference colours with viridis and saves as
-// PNG through OpenCV, which this build does not use).
+// PNG through OpenCV, which this build does not use).  Like the reference's,
+// a frame holds the state before that step's ghost fill.
 #pragma once
 
 #include <algorithm>
@@ -144,10 +145,10 @@ class RGRP_2D_CUDA {
     }
 
     // The time loop on the device in chunks of gap_ steps; after the first
-    // step and every gap_ further steps the state is copied out and its
-    // density written as <recordDir_>/<step index>.pgm.  The frame holds the
-    // state after that step's ghost fill (the reference copies it out just
-    // before the fill).
+    // step and every gap_ further steps the state is copied out as the
+    // reference copies it (after the branch average, before the ghost fill:
+    // fvm2d_download_frame) and its density written as
+    // <recordDir_>/<step index>.pgm.
     void solve_recording(const fvm2d_params &p, int *steps_out, double *ms_out) {
         fvm2d_solver *sv = nullptr;
         check(fvm2d_create(Nx_, Ny_, &p, &sv), "fvm2d_create");
@@ -166,7 +167,8 @@ class RGRP_2D_CUDA {
                 check(fvm2d_sync(sv, &steps, &t, &dt), "fvm2d_sync");
                 if (steps < due) break;
                 done = steps;
-                check(fvm2d_download(sv, reinterpret_cast<double *>(frame.data())), "fvm2d_download");
+                check(fvm2d_download_frame(sv, reinterpret_cast<double *>(frame.data())),
+                      "fvm2d_download_frame");
                 write_frame(frame, due - 1);
                 due += gap;
             }

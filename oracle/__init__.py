@@ -57,6 +57,9 @@ def lib():
         L.orc_solve.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcParams),
                                 ctypes.c_int, ip, dp, dp]
         L.orc_solve.restype = ctypes.c_int
+        L.orc_solve_record.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcParams),
+                                       ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip, ip, dp]
+        L.orc_solve_record.restype = ctypes.c_int
         L.orc_slope.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_double, dp]
         L.orc_grp.argtypes = [dp, dp, dp, dp, ctypes.c_double, dp, dp, ip]
         L.orc_flux.argtypes = [dp, dp, ctypes.c_double, dp]
@@ -102,6 +105,29 @@ def solve(U5: np.ndarray, nthreads: int = 0, **kw):
     if rc != 0:
         raise RuntimeError(f"orc_solve failed ({rc})")
     return U, steps.value, t.value, ms.value
+
+
+def solve_record(U5: np.ndarray, gap: int, max_frames: int = 64, nthreads: int = 0, **kw):
+    """Record-mode solve (src/wrappers.cuh:253-261) on a copy of ``U5``.
+
+    Returns (U_final, steps, t_final, frames, indices): frames[k] is the
+    [ny][nx][5] state copied out for the frame with file index indices[k]
+    (after the average, before the ghost fill of that step)."""
+    U = np.ascontiguousarray(U5, dtype=np.float64).copy()
+    ny, nx = U.shape[0], U.shape[1]
+    frames = np.zeros((max_frames, ny, nx, 5))
+    idx = np.zeros(max_frames, dtype=np.int32)
+    nf, steps = ctypes.c_int(0), ctypes.c_int(0)
+    t = ctypes.c_double(0.0)
+    p = params(**kw)
+    rc = lib().orc_solve_record(_dp(U), nx, ny, ctypes.byref(p), int(nthreads), int(gap), _dp(frames),
+                                int(max_frames), ctypes.byref(nf),
+                                idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(steps),
+                                ctypes.byref(t))
+    if rc != 0:
+        raise RuntimeError(f"orc_solve_record failed ({rc})")
+    k = min(nf.value, max_frames)
+    return U, steps.value, t.value, frames[:k], idx[:k].tolist()
 
 
 def slope(qL, q, qR, h, alpha):

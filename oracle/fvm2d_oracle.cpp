@@ -390,8 +390,13 @@ void sweep_x(V5 *U, V5 *Us, V5 *Ulr, V5 *uut, int Nx, int Ny, double width, doub
 
 extern "C" {
 
-int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads, int *steps_out,
-              double *t_out, double *ms_out) {
+// wrappers.cuh:154-306.  gap > 0 adds record mode (wrappers.cuh:253-261):
+// the state is copied out after the average and before the ghost fill of
+// the first step and of every gap-th step after it, under the index
+// counter iter_ of the reference.
+static int solve_impl(double *U5, int nx, int ny, const orc_params *p, int nthreads, int gap,
+                      double *frames5, int max_frames, int *nframes_out, int *frame_index,
+                      int *steps_out, double *t_out, double *ms_out) {
     if (!U5 || !p || nx < 1 || ny < 1) return -1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -406,6 +411,7 @@ int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads, int
     const double W = p->width, H = p->height, CFL = p->cfl, al = p->alpha, g = p->gamma;
     double cTime = 0.0;
     int steps = 0;
+    int iter = 0, iter_ = 0, nframes = 0;
     auto start = std::chrono::steady_clock::now();
     // wrappers.cuh:196-284
     while (cTime < p->t_end) {
@@ -435,13 +441,40 @@ int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads, int
         k_combine(u, ucopy.data(), (int)N);
         cTime += dt;
         steps++;
+        if (gap > 0) {
+            if (iter == 0 || iter == gap) {
+                if (frames5 && nframes < max_frames) {
+                    std::memcpy(frames5 + (size_t)nframes * N * 5, u, N * sizeof(V5));
+                    if (frame_index) frame_index[nframes] = iter_;
+                }
+                nframes++;
+                iter = 0;
+            }
+        }
+        iter++;
+        iter_++;
         k_fill(u, Nx, Ny, p->strategy, W, H, cTime);
     }
     auto end = std::chrono::steady_clock::now();
+    if (nframes_out) *nframes_out = nframes;
     if (steps_out) *steps_out = steps;
     if (t_out) *t_out = cTime;
     if (ms_out) *ms_out = std::chrono::duration<double, std::milli>(end - start).count();
     return 0;
+}
+
+int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads, int *steps_out,
+              double *t_out, double *ms_out) {
+    return solve_impl(U5, nx, ny, p, nthreads, 0, nullptr, 0, nullptr, nullptr, steps_out, t_out,
+                      ms_out);
+}
+
+int orc_solve_record(double *U5, int nx, int ny, const orc_params *p, int nthreads, int gap,
+                     double *frames5, int max_frames, int *nframes_out, int *frame_index,
+                     int *steps_out, double *t_out) {
+    if (gap < 1) return -1;
+    return solve_impl(U5, nx, ny, p, nthreads, gap, frames5, max_frames, nframes_out, frame_index,
+                      steps_out, t_out, nullptr);
 }
 
 void orc_slope(const double *qL, const double *q, const double *qR, double h, double alpha,

@@ -44,6 +44,15 @@ typedef struct {
 int orc_solve(double *U5, int nx, int ny, const orc_params *p, int nthreads,
               int *steps_out, double *t_out, double *ms_out);
 
+/* Record mode (wrappers.cuh:253-261, setRecordModel(true, gap)): the same
+ * solve, and the state as the reference copies it for a frame -- after the
+ * branch average, before the ghost fill -- at the first step and every gap
+ * steps after it: frames5 receives up to max_frames [ny][nx][5] states,
+ * frame_index their file indices; *nframes_out counts every frame taken. */
+int orc_solve_record(double *U5, int nx, int ny, const orc_params *p, int nthreads, int gap,
+                     double *frames5, int max_frames, int *nframes_out, int *frame_index,
+                     int *steps_out, double *t_out);
+
 /* Per-routine entry points for unit tests (line order: left, centre, right). */
 void orc_slope(const double *qL, const double *q, const double *qR, double h,
                double alpha, double *sigma4);

@@ -1,8 +1,9 @@
 // test2D-style caller (test2D.cu:46-80, 135-152) of the drop-in header.
-// Usage: drop_in_caller N t_end [mode [record_dir gap]]
-//   -> prints "steps ms rho_sum"
+// Usage: drop_in_caller N t_end [mode [record_dir|- gap [txt_dir]]]
+//   -> prints "steps ms rho_sum"; with txt_dir, writeFinalResultTo_txt(txt_dir)
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 #include <vector>
 
@@ -32,12 +33,13 @@ int main(int argc, char **argv) {
     solver.setAlpha(1.9);
     solver.setGamma(1.4);
     if (argc > 3) solver.setArithmeticMode(std::atoi(argv[3]));
-    if (argc > 5) solver.setRecordModel(true, std::atoi(argv[5]), argv[4]);
+    if (argc > 5 && std::strcmp(argv[4], "-") != 0) solver.setRecordModel(true, std::atoi(argv[5]), argv[4]);
     auto U = solver.solve();
     double s = 0.0;
     for (auto &row : U)
         for (auto &c : row) s += c(0);
     std::printf("%d %.3f %.17g\n", solver.getTimeStagesCount(), solver.getComputationTime_ms(), s);
+    if (argc > 6) solver.writeFinalResultTo_txt(argv[6]);
     solver.clear();
     return 0;
 }

@@ -1,6 +1,7 @@
 """GPU adjoint (fvm2d_adj_*) against the FP64 host reference gradient,
 central finite differences of the forward, and its own invariants.
-States and gradients cross the C ABI in host memory (numpy / CPU tensors)."""
+States and gradients cross the C ABI in host memory (numpy / CPU tensors)
+or, for CUDA tensors, as device pointers."""
 import numpy as np
 import pytest
 
@@ -263,3 +264,49 @@ def test_batched_adjoint_matches_single(gpu, problems, adjmod):
         assert np.array_equal(g0, g0s[i])
         one.close()
     adj.close()
+
+
+DEVICE_SCRIPT = r"""
+import importlib, sys
+import torch                                  # first: its HIP runtime is the only one
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+fvm = importlib.import_module("2d_fvm_cuda_amd")
+fvm.lib()
+assert len(fvm.hip_runtimes()) == 1, fvm.hip_runtimes()
+adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
+from test_gpu_adjoint import smooth_state
+shapes = [(40, 52), (36, 36)]
+Cs = [smooth_state(fvm.problems, nx, ny, seed=ny + 1) for ny, nx in shapes]
+ws = [np.random.default_rng(k).normal(size=C.shape) for k, C in enumerate(Cs)]
+adj = adjmod.Adjoint(shapes, [{}] * 2, nsteps=7, checkpoint_slots=3)
+xs = [torch.from_numpy(C.copy()).cuda().requires_grad_(True) for C in Cs]
+UNs = adjmod.FVMSolve.apply(adj, *xs)
+assert all(u.is_cuda for u in UNs)
+sum((u * torch.from_numpy(w).cuda()).sum() for u, w in zip(UNs, ws)).backward()
+host_UN = adj.forward(Cs)
+host_g = adj.backward(ws)
+adj.close()
+for x, u, hu, hg in zip(xs, UNs, host_UN, host_g):
+    assert x.grad.is_cuda
+    assert np.array_equal(u.detach().cpu().numpy(), hu)
+    assert np.array_equal(x.grad.cpu().numpy(), hg)
+print("DEVICE_TENSORS_OK", fvm.hip_runtimes())
+"""
+
+
+def test_autograd_on_device_tensors_matches_host(gpu):
+    """With torch imported before libfvm2d (one HIP runtime in the process),
+    FVMSolve on CUDA tensors -- device pointers through fvm2d_adj_forward /
+    fvm2d_adj_backward, no host staging -- gives the host path's states and
+    gradients bit for bit, for a batch of two meshes.  Runs in a child
+    process, where torch can be imported first."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", DEVICE_SCRIPT, root], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "DEVICE_TENSORS_OK" in r.stdout

@@ -38,3 +38,46 @@ def test_matrix_txt_names_and_format(tmp_path):
     assert txt == "1 2.5 0.333333 \n4 5 6 \n"
     for name in ("p", "u", "v"):
         assert os.path.exists(tmp_path / f"{name}_0.300000s.txt")
+
+
+def _dt(orc, U, t, nx, ny, p):
+    """dt rule of src/wrappers.cuh:198-203 on state U at time t."""
+    smax = max(orc.speed(U[j, i, :4], p["gamma"]) for j in range(ny) for i in range(nx))
+    h = min(p["width"] / nx, p["height"] / ny)
+    return min(p["cfl"] * h / smax, p["t_end"] - t)
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("strategy", [0, 4])
+def test_oracle_record_frames_are_the_prefill_states(orc, problems, strategy):
+    """The oracle's record mode (src/wrappers.cuh:253-261) copies the state
+    between the branch average and the ghost fill: filling a frame gives the
+    state after its step; its ghost rows are the X-sweep of the step's input
+    (the Y-sweep leaves them alone), its ghost columns the Y-sweep (the
+    X-sweep leaves them alone) and its corners the input itself.  Frames
+    come after the first step and every gap steps, indexed 0, gap, 2 gap..."""
+    rng = problems.SplitMix64(19 + strategy)
+    ny, nx = 28, 36
+    U0 = problems.random_four_quadrant(rng, nx, ny)
+    p = dict(width=1.0, height=1.0, t_end=0.05, cfl=0.5, alpha=1.9, gamma=1.4, strategy=strategy)
+    U, n, t, frames, idx = orc.solve_record(U0, gap=2, nthreads=1, **p)
+    assert n >= 5 and idx == list(range(0, n, 2))
+    for frame, k in zip(frames, idx):
+        s = k + 1  # steps done when the frame is taken
+        after, ns, ts, _ = orc.solve(U0, nthreads=1, max_steps=s, **p)
+        prev, _, tp, _ = orc.solve(U0, nthreads=1, max_steps=s - 1, **p)
+        assert ns == s
+        filled = orc.fill(frame, strategy, ts, p["width"], p["height"])
+        assert np.array_equal(filled[..., :4], after[..., :4])
+        dt = _dt(orc, prev, tp, nx, ny, p)
+        X = orc.sweep_x(prev, p["width"], dt, p["alpha"], p["gamma"])
+        Y = orc.rotate(orc.sweep_x(orc.rotate(prev), p["height"], dt, p["alpha"], p["gamma"]))
+        g = [0, 1, ny - 2, ny - 1]
+        c = [0, 1, nx - 2, nx - 1]
+        assert np.array_equal(frame[g, 2:nx - 2, :4], X[g, 2:nx - 2, :4])
+        assert np.array_equal(frame[2:ny - 2][:, c, :4], Y[2:ny - 2][:, c, :4])
+        assert np.array_equal(frame[np.ix_(g, c)][..., :4], prev[np.ix_(g, c)][..., :4])
+        # the fill changes the ghost cells: the frame is not the filled state
+        assert not np.array_equal(frame[..., :4], after[..., :4])
