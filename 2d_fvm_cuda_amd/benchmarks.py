@@ -108,7 +108,7 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
         fd = float((0.5 * (a - b) * (a + b - 2 * target)).sum()) / (2 * h)
         devs.append(abs(fd - g[k, j, i]) / max(abs(fd), 1e-6 * np.abs(g).max()))
     adj.close()
-    devs = np.array(devs)
+    devs = np.array(devs) if devs else np.zeros(1)
     return {"mesh": [n, n], "steps": steps, "checkpoint_slots": slots, "loss": loss,
             "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
             "forward_step_ms": st["forward_step_ms"], "reverse_step_ms": st["reverse_step_ms"],

@@ -355,14 +355,14 @@ struct AdjEngine {
         AdjDesc *descs = d_atab + ((size_t)loc * 2 + (reverse_steps & 1)) * n;
         HIPCHK(hipEventRecord(ring0[k], fwd.stream));
         adj_launch_smax(descs, n, fwd.stream);
-        // one wave per SIMD for the sweep adjoints (they need > 256
-        // registers), four for the lighter forward recompute
+        // two resident waves per SIMD for the sweep adjoints (256
+        // registers each), four for the lighter forward recompute
         const int simds = 4 * fwd.cu_count;
         adj_launch_fwd(descs, d_tfwd, nfwd, 4 * simds, fwd.stream);
         adj_launch_fwd_y(descs, d_tfy, nfy, fwd.stream);
-        adj_launch_sweep(descs, d_tr2, nr2, 0, 0, simds, fwd.stream);
-        adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, simds, fwd.stream);
-        adj_launch_sweep(descs, d_tr3b, nr3b, 2, nr2 + nr3a, simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr2, nr2, 0, 0, 2 * simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, 2 * simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr3b, nr3b, 2, nr2 + nr3a, 2 * simds, fwd.stream);
         adj_launch_dt_term(descs, n, nr2 + nr3a + nr3b, fwd.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ring1[k], fwd.stream));

@@ -328,44 +328,13 @@ __global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ des
 // result written to lam_out.  mode 2 (R3b, Y tasks): input U, output adjoint
 // lam_B1, result added to lam_out.
 
-// The loads of one task, issued one task ahead: the input state at the
-// lane's cell, the output adjoint (mode 0: lam' at the cell; the fill^T
-// ghost additions come at use), and in mode 2 the value the result is
-// added to.
-struct SweepLoads {
-    Q q, qbo, acc;
-};
-
-__device__ __forceinline__ SweepLoads sweep_loads(const AdjDesc &d, const LineMap &m, int mode) {
-    const long long P = d.plane;
-    SweepLoads in;
-    in.q = load_line(mode == 0 ? (m.ydir ? d.A1 : d.B1) : d.U, P, m);
-    if (mode == 0) {
-        const long long o = off_of(m);
-        in.qbo = lam_at(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch));
-    } else {
-        in.qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
-    }
-    in.acc = mode == 2 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
-    return in;
-}
-
-__device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, const SweepLoads &in,
-                                           int lane, int mode, int part) {
+__device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, int lane, int mode, int part) {
     const Line L = line_of(d, m);
     const Gas &G = d.gas;
     const double dt = d.ctl->dt;
     const double lam = dt / L.h;
     const long long P = d.plane;
-
-    Q qbo = in.qbo;
-    if (mode == 0) {
-        const long long o = off_of(m);
-        const Q s = lamC_ghosts(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch), qbo);
-        const Q h{0.5 * s.r, 0.5 * s.p, 0.5 * s.un, 0.5 * s.ut};
-        qbo = pick(m.ydir, swap_uv(h), h);
-    }
-    const Q q = in.q;
+    const double *src = mode == 0 ? (m.ydir ? d.A1 : d.B1) : d.U;
 
     const bool inl = m.k >= 0 && m.k < m.n;
     const bool upd = inl && m.k >= 2 && m.k <= m.n - 3;
@@ -374,23 +343,39 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, c
     const bool own = inl && lane >= kAdjHalo && lane < kWave - kAdjHalo;
 
     // ---- forward recompute of the sweep
-    const Q qL = shr(q), qR = shl(q);
-    const Q s = slope<false>(qL, q, qR, L);
-    const Q fm = face_minus<false>(q, s, L);
-    const Q fp = face_plus<false>(q, s, L);
-    const Q fpL = shr(fp), sL = shr(s);
+    Q fpL, fm, sL, s;
     IfState W, Wt;
-    grp<false>(fpL, fm, sL, s, G, W, Wt);
-    const Q F = flux<false>(W, Wt, dt);
-    const Q FR = shl(F);
-    const Q qo = update<false>(q, F, FR, lam, G);
-
-    // ---- reverse
+    Q F, FR, qo;
+    {
+        const Q q = load_line(src, P, m);
+        const Q qL = shr(q), qR = shl(q);
+        s = slope<false>(qL, q, qR, L);
+        fm = face_minus<false>(q, s, L);
+        const Q fp = face_plus<false>(q, s, L);
+        fpL = shr(fp);
+        sL = shr(s);
+        grp<false>(fpL, fm, sL, s, G, W, Wt);
+        F = flux<false>(W, Wt, dt);
+        FR = shl(F);
+        qo = update<false>(q, F, FR, lam, G);
+    }
+    // ---- reverse: cell update and flux
+    Q qbo;
+    {
+        const long long o = off_of(m);
+        if (mode == 0) {
+            const Q s0 = lamC_ghosts(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch),
+                                     lam_at(d, d.lam_in, (int)(o / d.pitch), (int)(o % d.pitch)));
+            const Q h{0.5 * s0.r, 0.5 * s0.p, 0.5 * s0.un, 0.5 * s0.ut};
+            qbo = pick(m.ydir, swap_uv(h), h);
+        } else {
+            qbo = load_line(m.ydir ? d.lamB1 : d.lamA1, P, m);
+        }
+    }
     double dtb = 0.0;
     Q Ub{0.0, 0.0, 0.0, 0.0};
     if (upd) {
         Ub = prim_adj(qo, qbo, G);
-        // U = cons(q) - lam (FR - F), lam = dt / h
         dtb -= ((FR.r - F.r) * Ub.r + (FR.p - F.p) * Ub.p + (FR.un - F.un) * Ub.un +
                 (FR.ut - F.ut) * Ub.ut) * L.ih;
     }
@@ -404,6 +389,10 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, c
         dtb += flux_adj(W, Wt, dt, Fb, Wb, Wtb);
         grp_adj(fpL, fm, sL, s, G, Wb, Wtb, fpLb, fmb, sLb, sb);
     }
+    // the line state again (a reload, so that it is not held across grp_adj)
+    asm volatile("" ::: "memory");
+    const Q q = load_line(src, P, m);
+    const Q qL = shr(q), qR = shl(q);
     const Q fpb = shl(fpLb), sbR = shl(sLb);
     Q qb{fmb.r + fpb.r, fmb.p + fpb.p, fmb.un + fpb.un, fmb.ut + fpb.ut};
     Q st{0.0, 0.0, 0.0, 0.0};
@@ -415,9 +404,10 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, c
     slope_adj(qL, q, qR, L, st, pL, pC, pR);
     const Q fromR = shl(pL), fromL = shr(pR);
     const Q cb = pick(upd, cons_adj(q, Ub, G), qbo);
-    qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r + in.acc.r, qb.p + pC.p + fromR.p + fromL.p + cb.p + in.acc.p,
-           qb.un + pC.un + fromR.un + fromL.un + cb.un + in.acc.un,
-           qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut + in.acc.ut};
+    const Q acc = mode == 2 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
+    qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r + acc.r, qb.p + pC.p + fromR.p + fromL.p + cb.p + acc.p,
+           qb.un + pC.un + fromR.un + fromL.un + cb.un + acc.un,
+           qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut + acc.ut};
 
     double *out = mode == 0 ? (m.ydir ? d.lamA1 : d.lamB1) : d.lam_out;
     if (own) store_line(out, P, m, qb, false);
@@ -425,36 +415,21 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, c
     if (lane == 0) d.dtpart[part] = part_sum;
 }
 
-// Each wave walks a strided list of tasks and issues the next task's loads
-// before it computes the current one (one wave per SIMD fits: the sweep
-// adjoint needs more than 256 registers).
-__global__ void __launch_bounds__(256) adj_sweep_kernel(AdjDesc *__restrict__ descs,
+// Each wave walks a strided list of tasks.  Two waves per SIMD: nothing is
+// loaded ahead, and only the GRP adjoint's own inputs and outputs stay in
+// registers across it (the line state is loaded again after it), which
+// keeps the kernel within 256 registers without spills; the second wave
+// hides the latencies one wave alone would wait on.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) adj_sweep_kernel(AdjDesc *__restrict__ descs,
                                                         const int4 *__restrict__ tasks, int ntasks,
                                                         int mode, int part_base) {
     const int lane = threadIdx.x & (kWave - 1);
     const int nwv = gridDim.x * (blockDim.x / kWave);
-    int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-    if (tk >= ntasks) return;
-    int4 t = uniform_task(tasks, tk);
-    LineMap m = line_map(descs[t.x], t, lane, kAdjHalo, kAdjOwn);
-    SweepLoads in = sweep_loads(descs[t.x], m, mode);
-    for (;;) {
-        const int tn = tk + nwv;
-        const bool more = tn < ntasks;
-        int4 t2 = t;
-        LineMap m2 = m;
-        SweepLoads in2 = in;
-        if (more) {
-            t2 = uniform_task(tasks, tn);
-            m2 = line_map(descs[t2.x], t2, lane, kAdjHalo, kAdjOwn);
-            in2 = sweep_loads(descs[t2.x], m2, mode);
-        }
-        sweep_task(descs[t.x], m, in, lane, mode, part_base + tk);
-        if (!more) break;
-        tk = tn;
-        t = t2;
-        m = m2;
-        in = in2;
+    for (int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+         tk < ntasks; tk += nwv) {
+        const int4 t = uniform_task(tasks, tk);
+        const LineMap m = line_map(descs[t.x], t, lane, kAdjHalo, kAdjOwn);
+        sweep_task(descs[t.x], m, lane, mode, part_base + tk);
     }
 }
 
@@ -592,6 +567,7 @@ void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
 int adj_smax_pairs() { return kSmaxBlocks; }
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
                       int max_waves, hipStream_t s) {
+    // max_waves: the resident waves wanted (two per SIMD fit)
     const int waves = ntasks < max_waves ? ntasks : max_waves;
     hipLaunchKernelGGL(adj_sweep_kernel, dim3(blocks_for(waves, 4)), dim3(256), 0, s, descs, tasks,
                        ntasks, mode, part_base);

@@ -73,6 +73,8 @@ def main():
                     help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
     ap.add_argument("--leg", choices=["dropin", "batched", "adjoint", "batched_adjoint", "peak_8192"],
                     help="run only this extra leg and print its record")
+    ap.add_argument("--adj-fd-cells", type=int, default=50,
+                    help="cells of the adjoint leg's central-difference check (0 skips it)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the VALU throughput probe (instructions per SIMD per second)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -105,7 +107,8 @@ def main():
     if args.leg:
         extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
         fn = {"dropin": lambda f: extras.dropin(f, args.n, args.steps or -1, fvm.MODE_FAST),
-              "batched": extras.batched, "adjoint": extras.adjoint,
+              "batched": extras.batched,
+              "adjoint": lambda f: extras.adjoint(f, fd_cells=args.adj_fd_cells),
               "batched_adjoint": extras.batched_adjoint, "peak_8192": extras.peak}[args.leg]
         print(json.dumps({args.leg: fn(fvm)}))
         return
@@ -193,7 +196,8 @@ def main():
     if not args.no_extras:
         extras = importlib.import_module("2d_fvm_cuda_amd.benchmarks")
         legs = (("dropin", lambda f: extras.dropin(f, N, steps, mode, out["ms_per_step"])),
-                ("batched", extras.batched), ("adjoint", extras.adjoint),
+                ("batched", extras.batched),
+                ("adjoint", lambda f: extras.adjoint(f, fd_cells=args.adj_fd_cells)),
                 ("batched_adjoint", extras.batched_adjoint), ("peak_8192", extras.peak))
         for key, fn in legs:
             try:
