@@ -25,9 +25,12 @@ struct AdjEngine {
     // per instance device buffers
     std::vector<double *> ck;        // [inst * slots + s] checkpoint states
     std::vector<double *> A1, B1, lamA1, lamB1, lamX, lamY, UN, U0c;
-    std::vector<unsigned long long *> pmax;
-    std::vector<long long *> pidx;
     double *dtpart = nullptr;
+    // state locations (the slots, then the engine's buf0 and buf1) as
+    // device tables of n instance pointers each, for one-launch copies
+    double **d_locptr = nullptr;
+    long long *d_counts = nullptr;
+    long long max_count = 0;
     AdjCtl *d_actl = nullptr;
     std::vector<AdjDesc> had;
     int4 *d_tfwd = nullptr, *d_tfy = nullptr, *d_tr2 = nullptr, *d_tr3a = nullptr, *d_tr3b = nullptr;
@@ -99,7 +102,7 @@ struct AdjEngine {
         had.resize(n);
         ck.resize((size_t)n * slots);
         A1.resize(n), B1.resize(n), lamA1.resize(n), lamB1.resize(n), lamX.resize(n), lamY.resize(n);
-        UN.resize(n), U0c.resize(n), pmax.resize(n), pidx.resize(n);
+        UN.resize(n), U0c.resize(n);
         rc = dalloc(&d_actl, n);
         if (rc) return rc;
         for (int i = 0; i < n; i++) {
@@ -110,9 +113,6 @@ struct AdjEngine {
             for (double **b : {&A1[i], &B1[i], &lamA1[i], &lamB1[i], &lamX[i], &lamY[i], &UN[i]})
                 if ((rc = dalloc(b, st))) return rc;
             if ((rc = dalloc(&U0c[i], 4 * (size_t)nx[i] * ny[i]))) return rc;
-            const size_t npairs = (size_t)adj_smax_pairs();
-            if ((rc = dalloc(&pmax[i], npairs))) return rc;
-            if ((rc = dalloc(&pidx[i], npairs))) return rc;
             HIPCHK(hipMemsetAsync(A1[i], 0, st * sizeof(double), fwd.stream));
             HIPCHK(hipMemsetAsync(B1[i], 0, st * sizeof(double), fwd.stream));
             AdjDesc &a = had[i];
@@ -130,8 +130,6 @@ struct AdjEngine {
             a.B1 = B1[i];
             a.lamA1 = lamA1[i];
             a.lamB1 = lamB1[i];
-            a.pair_max = pmax[i];
-            a.pair_idx = pidx[i];
             a.ctl = d_actl + i;
             gamma.push_back(p[i].gamma);
         }
@@ -139,7 +137,22 @@ struct AdjEngine {
         if (rc) return rc;
         rc = dalloc(&d_slot_ctl, (size_t)slots * n);
         if (!rc) rc = dalloc(&d_atab, (size_t)(slots + 2) * 2 * n);
+        if (!rc) rc = dalloc(&d_locptr, (size_t)(slots + 2) * n);
+        if (!rc) rc = dalloc(&d_counts, (size_t)n);
         if (rc) return rc;
+        {
+            std::vector<double *> lp((size_t)(slots + 2) * n);
+            std::vector<long long> cnt(n);
+            for (int loc = 0; loc < slots + 2; loc++)
+                for (int i = 0; i < n; i++)
+                    lp[(size_t)loc * n + i] = const_cast<double *>(state_ptr(i, loc_where(loc)));
+            for (int i = 0; i < n; i++) {
+                cnt[i] = 4 * fwd.hd[i].plane;
+                max_count = std::max(max_count, cnt[i]);
+            }
+            HIPCHK(hipMemcpy(d_locptr, lp.data(), lp.size() * sizeof(double *), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_counts, cnt.data(), cnt.size() * sizeof(long long), hipMemcpyHostToDevice));
+        }
         build_schedule();
         HIPCHK(hipStreamSynchronize(fwd.stream));
         return FVM2D_OK;
@@ -268,13 +281,19 @@ struct AdjEngine {
         return where == kBuf0 ? d.buf0 : where == kBuf1 ? d.buf1 : ck[(size_t)i * slots + where];
     }
 
-    // Copy the state of every instance from src to dst (locations as above).
+    // Location index in the pointer tables: slots first, then buf0, buf1.
+    int loc_index(int where) const { return where >= 0 ? where : (where == kBuf0 ? slots : slots + 1); }
+    static int loc_where_of(int loc, int nslots) {
+        return loc < nslots ? loc : (loc == nslots ? kBuf0 : kBuf1);
+    }
+    int loc_where(int loc) const { return loc_where_of(loc, slots); }
+
+    // Copy the state of every instance from src to dst (locations as
+    // above), all instances in one launch.
     int copy_state(int src, int dst) {
-        for (int i = 0; i < n; i++) {
-            const InstDesc &d = fwd.hd[i];
-            HIPCHK(hipMemcpyAsync(const_cast<double *>(state_ptr(i, dst)), state_ptr(i, src),
-                                  4 * d.plane * sizeof(double), hipMemcpyDeviceToDevice, fwd.stream));
-        }
+        adj_launch_copy(d_locptr + (size_t)loc_index(src) * n, d_locptr + (size_t)loc_index(dst) * n,
+                        d_counts, n, max_count, fwd.stream);
+        HIPCHK(hipGetLastError());
         return FVM2D_OK;
     }
 
@@ -314,7 +333,11 @@ struct AdjEngine {
             for (int par = 0; par < 2; par++)
                 for (int i = 0; i < n; i++) {
                     AdjDesc a = had[i];
-                    a.U = state_ptr(i, loc < slots ? loc : (loc == slots ? kBuf0 : kBuf1));
+                    a.U = state_ptr(i, loc_where(loc));
+                    // a stored state's control block was saved with it; a
+                    // recomputed one is the engine's current block (a
+                    // reverse step follows the advance that produced it)
+                    a.fctl = loc < slots ? d_slot_ctl + (size_t)loc * n + i : fwd.d_ctl + i;
                     a.lam_in = par ? lamY[i] : lamX[i];
                     a.lam_out = par ? lamX[i] : lamY[i];
                     tab[((size_t)loc * 2 + par) * n + i] = a;
@@ -354,15 +377,15 @@ struct AdjEngine {
         const int loc = s >= 0 ? s : (s == kBuf0 ? slots : slots + 1);
         AdjDesc *descs = d_atab + ((size_t)loc * 2 + (reverse_steps & 1)) * n;
         HIPCHK(hipEventRecord(ring0[k], fwd.stream));
-        adj_launch_smax(descs, n, fwd.stream);
         // two resident waves per SIMD for the sweep adjoints (256
-        // registers each), four for the lighter forward recompute
+        // registers each), four for the lighter forward recompute; the
+        // column sweep of R3 writes lam_out and the row sweep adds to it
         const int simds = 4 * fwd.cu_count;
         adj_launch_fwd(descs, d_tfwd, nfwd, 4 * simds, fwd.stream);
         adj_launch_fwd_y(descs, d_tfy, nfy, fwd.stream);
         adj_launch_sweep(descs, d_tr2, nr2, 0, 0, 2 * simds, fwd.stream);
-        adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, 2 * simds, fwd.stream);
         adj_launch_sweep(descs, d_tr3b, nr3b, 2, nr2 + nr3a, 2 * simds, fwd.stream);
+        adj_launch_sweep(descs, d_tr3a, nr3a, 1, nr2, 2 * simds, fwd.stream);
         adj_launch_dt_term(descs, n, nr2 + nr3a + nr3b, fwd.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ring1[k], fwd.stream));
@@ -500,7 +523,10 @@ struct AdjEngine {
         ring_pending = 0;
         int rc = build_desc_table();
         if (rc) return rc;
-        HIPCHK(hipMemsetAsync(d_actl, 0, n * sizeof(AdjCtl), fwd.stream));
+        {
+            std::vector<AdjCtl> init(n, AdjCtl{0.0, kNoCell, 0, 0});
+            HIPCHK(hipMemcpy(d_actl, init.data(), n * sizeof(AdjCtl), hipMemcpyHostToDevice));
+        }
         hipEvent_t b0, b1;
         HIPCHK(hipEventCreate(&b0));
         HIPCHK(hipEventCreate(&b1));

@@ -3,10 +3,12 @@
 // Forward step (fixed by the state U_n):  dt = CFL h / s_max(U_n)
 //     A1 = X(U), A = Y(A1);  B1 = Y(U), B = X(B1);  U' = fill((A + B)/2)
 // Reverse step, given lam' = dL/dU' (primitive planes, every cell):
-//   R1  recompute A1 = X(U), B1 = Y(U); s_max(U) with its arg-max cell, dt
+//   R1  recompute A1 = X(U), B1 = Y(U); s_max(U) and dt come from the forward
+//       time loop's control block of U, and the row sweeps find the first
+//       cell attaining s_max (its arg-max)
 //   R2  lam_A1 = Y^T(A1; lam_C/2),  lam_B1 = X^T(B1; lam_C/2)
 //       where lam_C = fill^T(lam') is gathered while loading
-//   R3  lam = X^T(U; lam_A1) + Y^T(U; lam_B1)
+//   R3  lam = Y^T(U; lam_B1) + X^T(U; lam_A1)
 //   R4  lam[argmax] += dtbar * d dt / d U   (dt depends on the arg-max cell)
 // Every sweep adjoint is one lane-parallel routine: lanes are 64
 // consecutive cells of one line; an X line is a row (unit stride), a Y line
@@ -17,6 +19,8 @@
 // hand-written adjoints of fvm2d_adjoint_cell.h; a cell's adjoint depends on
 // cells up to four away, so 56 of the 64 lanes produce output.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "fvm2d_adjoint_cell.h"
 #include "fvm2d_adjoint_kernels.h"
@@ -150,6 +154,18 @@ __device__ Q lamC_ghosts(const AdjDesc &d, const double *lam, int j, int c, Q s)
     return s;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b on XCD
+// b mod 8), and each XCD has its own L2.  Tasks of neighbouring columns read
+// the same cache lines (16 columns per 128-byte line), so a grid of a
+// multiple of 8 blocks is renumbered to give each XCD a contiguous range of
+// block indices, hence of tasks, instead of every 8th.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, n = gridDim.x;
+    if (n % kXcds) return b;
+    return (b % kXcds) * (n / kXcds) + b / kXcds;
+}
+
 // The task a wave works on is wave-uniform: keep it in scalar registers.
 __device__ __forceinline__ int4 uniform_task(const int4 *__restrict__ tasks, int tk) {
     const int4 v = tasks[tk];
@@ -183,11 +199,20 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
         }
         const AdjDesc &d = descs[t.x];
         const Line L = line_of(d, m);
-        const double dt = d.ctl->dt;
+        const double dt = d.fctl->dt;
         const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
         const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
         const bool upd = m.k >= 2 && m.k <= m.n - 3;
         if (own) store_line(m.ydir ? d.B1 : d.A1, d.plane, m, pick(upd, qo, q), false);
+        if (!m.ydir) {
+            // the X tasks cover every cell of U once: the first cell whose
+            // wave speed is s_max (the forward loop's maximum over all cells,
+            // by the same function) is the arg-max
+            const unsigned long long sb = bits_of(d.fctl->smax);
+            if (own && bits_of(speed<false>(q.r, q.p, q.un, q.ut, d.gas)) == sb)
+                atomicMin(&d.ctl->argmax, (long long)t.w * d.nx + m.k);
+            if (lane == 0 && !(sb <= 0x7ff0000000000000ull)) atomicOr(&d.ctl->status, 1);
+        }
         if (!more) break;
         tk = tn;
         t = t2;
@@ -217,7 +242,7 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
     const int col = t.y * kWave + lane;
     const int colc = col < nx ? col : nx - 1;
     const bool valid = col < nx;
-    const double dt = d.ctl->dt;
+    const double dt = d.fctl->dt;
     const Line LY = d.ly;
     const double lamy = dt / LY.h;
     auto load = [&](int j) -> Q {
@@ -251,87 +276,18 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
     }
 }
 
-constexpr int kSmaxBlocks = 256;  // most blocks per instance of the s_max kernel
-
-__device__ __forceinline__ void max_pair(unsigned long long &mx, long long &idx, unsigned long long m2,
-                                         long long i2) {
-    if (m2 > mx || (m2 == mx && i2 < idx)) {
-        mx = m2;
-        idx = i2;
-    }
-}
-
-__device__ __forceinline__ void wave_max_pair(unsigned long long &mx, long long &idx) {
-    for (int o = 32; o > 0; o >>= 1)
-        max_pair(mx, idx, __shfl_xor(mx, o, kWave), __shfl_xor(idx, o, kWave));
-}
-
-// s_max over every cell of U_n with the smallest linear index j * nx + i
-// attaining it (largest bit pattern, then smallest index: the result does
-// not depend on the reduction order), and dt = CFL h / s_max.  blockIdx.y is
-// the instance; each of its gridDim.x blocks strides over rows and leaves
-// one pair, and the last block to arrive reduces the pairs.
-__global__ void __launch_bounds__(256) adj_smax_kernel(AdjDesc *__restrict__ descs) {
-    __shared__ unsigned long long smx[256 / kWave];
-    __shared__ long long sidx[256 / kWave];
-    __shared__ int last;
-    AdjDesc &d = descs[blockIdx.y];
-    unsigned long long mx = 0ull;
-    long long idx = 0x7fffffffffffffffll;
-    for (int j = blockIdx.x; j < d.ny; j += gridDim.x) {
-        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
-            const long long o = (long long)j * d.pitch + i;
-            const double sp = speed<false>(d.U[o], d.U[d.plane + o], d.U[2 * d.plane + o],
-                                           d.U[3 * d.plane + o], d.gas);
-            max_pair(mx, idx, bits_of(sp), (long long)j * d.nx + i);
-        }
-    }
-    wave_max_pair(mx, idx);
-    const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    if (lane == 0) {
-        smx[wv] = mx;
-        sidx[wv] = idx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x / kWave); k++) max_pair(mx, idx, smx[k], sidx[k]);
-        d.pair_max[blockIdx.x] = mx;
-        d.pair_idx[blockIdx.x] = idx;
-        __threadfence();
-        last = atomicAdd(&d.ctl->arrivals, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last || wv != 0) return;
-    __threadfence();
-    mx = 0ull;
-    idx = 0x7fffffffffffffffll;
-    for (int k = lane; k < (int)gridDim.x; k += kWave) {
-        // written by other blocks of this launch: read past the caches
-        max_pair(mx, idx, __hip_atomic_load(&d.pair_max[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                 __hip_atomic_load(&d.pair_idx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    wave_max_pair(mx, idx);
-    if (lane == 0) {
-        const double smax = __longlong_as_double((long long)mx);
-        d.ctl->smax = smax;
-        d.ctl->argmax = idx;
-        d.ctl->dt = d.cfl * d.hmin / smax;
-        d.ctl->status |= (mx > 0x7ff0000000000000ull) ? 1 : 0;
-        d.ctl->arrivals = 0u;
-    }
-}
-
 // ---------------------------------------------------------------- R2 / R3
 // One sweep adjoint per task.  mode 0 (R2): input state A1 (Y tasks) or B1
 // (X tasks), output adjoint lam_C/2 gathered from lam_in, result written to
-// lam_A1 / lam_B1.  mode 1 (R3a, X tasks): input U, output adjoint lam_A1,
-// result written to lam_out.  mode 2 (R3b, Y tasks): input U, output adjoint
-// lam_B1, result added to lam_out.
+// lam_A1 / lam_B1.  mode 2 (R3b, Y tasks, launched first): input U, output
+// adjoint lam_B1, result written to lam_out.  mode 1 (R3a, X tasks): input
+// U, output adjoint lam_A1, result added to lam_out -- the accumulation's
+// extra load and store are row segments there, not column-strided.
 
 __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, int lane, int mode, int part) {
     const Line L = line_of(d, m);
     const Gas &G = d.gas;
-    const double dt = d.ctl->dt;
+    const double dt = d.fctl->dt;
     const double lam = dt / L.h;
     const long long P = d.plane;
     const double *src = mode == 0 ? (m.ydir ? d.A1 : d.B1) : d.U;
@@ -404,7 +360,7 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, i
     slope_adj(qL, q, qR, L, st, pL, pC, pR);
     const Q fromR = shl(pL), fromL = shr(pR);
     const Q cb = pick(upd, cons_adj(q, Ub, G), qbo);
-    const Q acc = mode == 2 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
+    const Q acc = mode == 1 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
     qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r + acc.r, qb.p + pC.p + fromR.p + fromL.p + cb.p + acc.p,
            qb.un + pC.un + fromR.un + fromL.un + cb.un + acc.un,
            qb.ut + pC.ut + fromR.ut + fromL.ut + cb.ut + acc.ut};
@@ -425,7 +381,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                                                         int mode, int part_base) {
     const int lane = threadIdx.x & (kWave - 1);
     const int nwv = gridDim.x * (blockDim.x / kWave);
-    for (int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+    for (int tk = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave);
          tk < ntasks; tk += nwv) {
         const int4 t = uniform_task(tasks, tk);
         const LineMap m = line_map(descs[t.x], t, lane, kAdjHalo, kAdjOwn);
@@ -469,12 +425,13 @@ __global__ void __launch_bounds__(1024) adj_dt_term_kernel(AdjDesc *__restrict__
         const long long o = (long long)j * d.pitch + i;
         const long long P = d.plane;
         const Q g = speed_grad(d.U[o], d.U[P + o], d.U[2 * P + o], d.U[3 * P + o], d.gas);
-        const double f = dtb * (-d.ctl->dt / d.ctl->smax);
+        const double f = dtb * (-d.fctl->dt / d.fctl->smax);
         d.lam_out[o] += f * g.r;
         d.lam_out[P + o] += f * g.p;
         d.lam_out[2 * P + o] += f * g.un;
         d.lam_out[3 * P + o] += f * g.ut;
         d.ctl->dtbar = dtb;
+        d.ctl->argmax = kNoCell;  // for the next reverse step's search
     }
 }
 
@@ -556,15 +513,23 @@ void adj_launch_fwd_y(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t
         hipLaunchKernelGGL(adj_fwd_y_kernel, dim3(blocks_for(ntasks, 4)), dim3(256), 0, s, descs,
                            tasks, ntasks);
 }
-void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s) {
-    // about 1024 blocks over the whole batch: at 32 meshes of 256-768 rows,
-    // 256 blocks per mesh left each block a row or two and made the
-    // reduction tail dominate (backward 5 % longer than with 32 per mesh);
-    // a single 512 x 512 mesh is fastest with 256
-    const int per = std::max(32, std::min(kSmaxBlocks, 1024 / std::max(1, ninst)));
-    hipLaunchKernelGGL(adj_smax_kernel, dim3(per, ninst), dim3(256), 0, s, descs);
+__global__ void __launch_bounds__(256) adj_copy_kernel(const double *const *__restrict__ src,
+                                                       double *const *__restrict__ dst,
+                                                       const long long *__restrict__ count) {
+    const int i = blockIdx.y;
+    const double2 *__restrict__ a = reinterpret_cast<const double2 *>(src[i]);
+    double2 *__restrict__ b = reinterpret_cast<double2 *>(dst[i]);
+    const long long n2 = count[i] / 2;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n2;
+         k += (long long)gridDim.x * blockDim.x)
+        b[k] = a[k];
 }
-int adj_smax_pairs() { return kSmaxBlocks; }
+
+void adj_launch_copy(const double *const *src, double *const *dst, const long long *count, int n,
+                     long long max_count, hipStream_t s) {
+    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(64, (max_count / 2 + 255) / 256));
+    hipLaunchKernelGGL(adj_copy_kernel, dim3(bx, n), dim3(256), 0, s, src, dst, count);
+}
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
                       int max_waves, hipStream_t s) {
     // max_waves: the resident waves wanted (two per SIMD fit)

@@ -9,13 +9,13 @@ namespace fvm2d {
 
 // Per-instance scalars of one reverse step, resident on the device.
 struct AdjCtl {
-    double smax;       // s_max of U_n
-    double dt;         // dt_n = CFL h / s_max
     double dtbar;      // total sensitivity of the loss to dt_n
     long long argmax;  // linear index (j * nx + i) of the first cell attaining s_max
+                       // (kNoCell between reverse steps)
     int status;        // bit 0: non-finite wave speed
-    unsigned int arrivals;  // blocks of the s_max kernel done (0 between steps)
+    int pad;
 };
+constexpr long long kNoCell = 0x7fffffffffffffffll;
 
 // Task of the line kernels: x = instance, y = 0 for an X line (a row) or 1
 // for a Y line (a column), z = window along the line, w = row or column.
@@ -26,13 +26,13 @@ struct AdjDesc {
     Line lx, ly;
     Gas gas;
     const double *U;       // state U_n (pitched primitive planes)
+    const Ctl *fctl;       // the forward time loop's control block of U_n: its
+                           // s_max and dt_n = CFL h / s_max (no t_end clamp)
     double *A1, *B1;       // first sweeps of the two branches, recomputed
     const double *lam_in;  // dL/dU_{n+1}
     double *lamA1, *lamB1; // dL/dA1, dL/dB1
     double *lam_out;       // dL/dU_n
     double *dtpart;        // per-task dt sensitivities (shared array)
-    unsigned long long *pair_max;  // per-wave s_max bit patterns
-    long long *pair_idx;           // per-wave arg-max indices
     AdjCtl *ctl;
     int part_lo[3], part_hi[3];    // this instance's ranges in dtpart
 };
@@ -42,10 +42,10 @@ void adj_launch_fwd(AdjDesc *descs, const int4 *tasks, int ntasks, int max_waves
 // B1 = Y(U) by the register march; tasks {instance, 64-column window, r0, r1}
 // tile every instance's interior rows [2, ny-2) for every window.
 void adj_launch_fwd_y(AdjDesc *descs, const int4 *tasks, int ntasks, hipStream_t s);
-// s_max, its arg-max and dt of every instance (one launch); the pair arrays
-// hold adj_smax_pairs() entries per instance.
-void adj_launch_smax(AdjDesc *descs, int ninst, hipStream_t s);
-int adj_smax_pairs();
+// Copy the states of all n instances from one location to another (one
+// launch): src[i] -> dst[i], count[i] doubles each (even, 16-byte aligned).
+void adj_launch_copy(const double *const *src, double *const *dst, const long long *count, int n,
+                     long long max_count, hipStream_t s);
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
                       int max_waves, hipStream_t s);
 void adj_launch_dt_term(AdjDesc *descs, int ninst, int nparts, hipStream_t s);
