@@ -1,0 +1,55 @@
+"""Writes config3_batch64.json (+ config3_batch64_sub.npz): the oracle's
+solve of each of the 64 batched-workload meshes (problems.batched_instances
+with seed 2024: sides ~U{128..1024}, non-square, seeded four-quadrant data,
+gamma 1.4, CFL 0.5, alpha 1.9, outflow), every mesh with its own dt for
+exactly 200 steps.  Per mesh: shape, step count, final time, SHA-256 of the
+final (rho, p, u, v) planes in [ny][nx][4] float64 order, and every 32nd
+cell of the final state.  About 4.4e9 cell-updates: ~15 minutes on 8 cores.
+    python tests/golden/make_config3.py [CACHE_DIR]
+With CACHE_DIR, each mesh's result is kept there as it is computed and a
+rerun continues from the meshes already done.
+"""
+import hashlib
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+
+problems = importlib.import_module("2d_fvm_cuda_amd.problems")
+HERE = os.path.dirname(os.path.abspath(__file__))
+COUNT, SEED, STEPS, STRIDE = 64, 2024, 200, 32
+
+CACHE = sys.argv[1] if len(sys.argv) > 1 else None
+if CACHE:
+    os.makedirs(CACHE, exist_ok=True)
+
+t0 = time.time()
+recs, subs = [], {}
+for i, U0 in enumerate(problems.batched_instances(count=COUNT, seed=SEED)):
+    cached = os.path.join(CACHE, f"m{i}.json") if CACHE else None
+    if cached and os.path.exists(cached):
+        recs.append(json.load(open(cached)))
+        subs[f"m{i}"] = np.load(cached[:-5] + ".npy")
+        continue
+    U, steps, t, ms = oracle.solve(U0, nthreads=0, t_end=1e30, max_steps=STEPS)
+    P4 = np.ascontiguousarray(U[..., :4])
+    recs.append({"shape": [int(U0.shape[0]), int(U0.shape[1])], "steps": steps, "t_final": t,
+                 "sha256_prim4": hashlib.sha256(P4.tobytes()).hexdigest()})
+    subs[f"m{i}"] = np.ascontiguousarray(P4[::STRIDE, ::STRIDE])
+    if cached:
+        np.save(cached[:-5] + ".npy", subs[f"m{i}"])
+        json.dump(recs[-1], open(cached, "w"))
+    print(i, U0.shape, steps, f"{time.time() - t0:.0f}s", flush=True)
+out = {"workload": f"{COUNT} meshes of problems.batched_instances(seed={SEED}), {STEPS} steps each, "
+                   "CFL 0.5, alpha 1.9, gamma 1.4, outflow",
+       "count": COUNT, "seed": SEED, "steps": STEPS, "stride": STRIDE, "meshes": recs}
+np.savez_compressed(os.path.join(HERE, "config3_batch64_sub.npz"), **subs)
+json.dump(out, open(os.path.join(HERE, "config3_batch64.json"), "w"), indent=1)
+print("wall", time.time() - t0)
