@@ -62,6 +62,7 @@ class Params(ctypes.Structure):
 
 EXPORTS = [
     "fvm2d_params_default", "fvm2d_last_error", "fvm2d_device_available", "fvm2d_solve",
+    "fvm2d_solve_device_noalloc",
     "fvm2d_create", "fvm2d_set_params", "fvm2d_upload", "fvm2d_upload_planes_device",
     "fvm2d_download_planes_device", "fvm2d_run", "fvm2d_download", "fvm2d_download_frame",
     "fvm2d_steps_total",
@@ -111,6 +112,7 @@ def lib():
         "fvm2d_last_error": (ctypes.c_char_p, []),
         "fvm2d_device_available": (ctypes.c_int, []),
         "fvm2d_solve": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, P, _ip, _dp]),
+        "fvm2d_solve_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, P, _ip, _dp]),
         "fvm2d_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(_vp)]),
         "fvm2d_set_params": (ctypes.c_int, [_vp, P]),
         "fvm2d_upload": (ctypes.c_int, [_vp, _dp]),

@@ -17,7 +17,8 @@ LIB = os.path.join(PKG, "libfvm2d.so")
 PROBES_LIB = os.path.join(PKG, "libfvm2d_probes.so")
 
 # the solver library, and the measurement probes kept in a library of their own
-SOURCES = ["fvm2d_step.hip", "fvm2d_adjoint.hip", "fvm2d_runtime.hip", "fvm2d_adj_runtime.hip"]
+SOURCES = ["fvm2d_step.hip", "fvm2d_adjoint.hip", "fvm2d_runtime.hip", "fvm2d_adj_runtime.hip",
+           "fvm2d_noalloc.hip"]
 PROBE_SOURCES = ["fvm2d_probes.hip"]
 HEADERS = ["fvm2d_cell.h", "fvm2d_device.h", "fvm2d_kernels.h", "fvm2d_adjoint_cell.h",
            "fvm2d_march.h"]

@@ -72,6 +72,18 @@ int fvm2d_device_available(void);
 int fvm2d_solve(double *U5, int nx, int ny, const fvm2d_params *p, int *steps_out,
                 double *ms_out);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (stream-capture set-up, serving loops, harnesses that forbid
+ * hipMalloc).  The solve loop of fvm2d_solve run in place on the caller's
+ * device planes [4][ny][nx] (rho, p, u, v, contiguous, row 0 = north,
+ * ghosts included), with every other buffer in a library-owned device array
+ * of fixed size (33 MiB per GPU, held until the process ends): no
+ * hipMalloc, hipFree or other allocating call.  Meshes up to 1048576 cells
+ * (larger: FVM2D_ERR_ARG).  Blocking, on the null stream.  All calls on one
+ * GPU share that array: calls that overlap on one GPU race. */
+int fvm2d_solve_device_noalloc(double *d_planes, int nx, int ny, const fvm2d_params *p,
+                               int *steps_out, double *t_out);
+
 /* ---- resident solver: the state stays in HBM between calls ------------- */
 typedef struct fvm2d_solver fvm2d_solver;
 
