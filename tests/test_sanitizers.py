@@ -1,0 +1,38 @@
+"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the
+oracle (every entry point, every preset, sizes that are not multiples of
+16, the host reference adjoint) and the engine's launch-layout logic and
+parameter checks, each built with g++ -fsanitize=address,undefined from a
+driver in tests/cpp/ and run; any report aborts the driver."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g", "-O1"]
+
+
+def _build_and_run(tmp_path, name, srcs, extra):
+    exe = str(tmp_path / name)
+    r = subprocess.run(["g++", "-std=c++17", *SAN, *extra, *srcs, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, ASAN_OPTIONS="abort_on_error=1:detect_leaks=1", OMP_NUM_THREADS="2",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-4000:])
+    assert r.returncode == 0 and r.stdout.startswith("OK")
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    srcs = [os.path.join(ROOT, "tests", "cpp", "oracle_sanitize.cpp"),
+            os.path.join(ROOT, "oracle", "fvm2d_oracle.cpp"),
+            os.path.join(ROOT, "oracle", "fvm2d_oracle_adjoint.cpp")]
+    _build_and_run(tmp_path, "oracle_san", srcs, ["-ffp-contract=off", "-fopenmp", "-I", os.path.join(ROOT, "oracle")])
+
+
+def test_engine_layout_under_sanitizers(tmp_path):
+    if not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"):
+        pytest.skip("HIP headers not installed")
+    srcs = [os.path.join(ROOT, "tests", "cpp", "layout_sanitize.cpp")]
+    _build_and_run(tmp_path, "layout_san", srcs, ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                                                  "-Wno-unused-result"])
