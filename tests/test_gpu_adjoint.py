@@ -213,6 +213,38 @@ def test_adjoint_workload_central_fd(gpu, problems, adjmod):
     adj.close()
 
 
+def test_adjoint512_tie_free_vs_host_reference(gpu, problems, adjmod):
+    """The 512^2, 400-step adjoint workload made tie-free (8 seeded
+    random-phase modes on each of u and v; tests/golden/make_adjoint512.py)
+    against the host reference (the oracle's FP64 adjoint, IEEE operations)
+    of L = <gN, U_400>: every 8th cell of U_400 and of dL/dU_0, each within
+    1e-10 of its largest magnitude."""
+    import importlib.util
+    import json
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_adjoint512", os.path.join(gdir, "make_adjoint512.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    golden = json.load(open(os.path.join(gdir, "adjoint512.json")))
+    ref = np.load(os.path.join(gdir, "adjoint512_sub8.npz"))
+    s = golden["stride"]
+    C0, gN = mk.workload()
+    print("512^2 tie-free adjoint, 400 steps", flush=True)
+    adj = adjmod.Adjoint([(512, 512)], [{}], nsteps=golden["steps"], checkpoint_slots=32)
+    UN = adj.forward([C0])[0]
+    g = adj.backward([gN])[0]
+    adj.close()
+    du = np.abs(UN[:, ::s, ::s] - ref["UN"]).max() / np.abs(ref["UN"]).max()
+    dg = np.abs(g[:, ::s, ::s] - ref["grad"]).max() / golden["grad_max_abs"]
+    dmax = abs(np.abs(g).max() - golden["grad_max_abs"]) / golden["grad_max_abs"]
+    print(f"U_400 max deviation {du:.2e}, gradient max deviation {dg:.2e} (of max |g|), "
+          f"max |g| deviation {dmax:.2e}")
+    assert du <= 1e-10
+    assert dg <= 1e-10
+    assert dmax <= 1e-10
+
+
 def test_autograd_matches_direct_backward(gpu, problems, adjmod):
     import torch
     C0 = smooth_state(problems, 40, 40, seed=4)
