@@ -11,8 +11,13 @@ adjoint()          512^2 Fourier-perturbed state, 400 steps forward with
                    gradient; reports the reverse-step / forward-step time ratio
                    and central differences (h = 1e-6) on 50 seeded cells.
 batched_adjoint()  32 meshes with sides ~U{256..768} (the adjoint recipe under
-                   seeds 1..32), 300 steps forward + backward in one batch.
+                   seeds 1..32), 300 steps forward + backward in one batch;
+                   recipe="riemann": the batched-forward recipe instead (seeded
+                   four-quadrant Riemann data per mesh, sides ~U{256..768}).
 peak()             8192^2 plain forward (2.1 GB of state), 20 timed steps.
+published()        the runs whose wall-clock times the reference publishes
+                   (README.md:2, test2D.cu:137-144, its figure's panels (a)
+                   and (c)), solved by the drop-in entry point.
 Cell-update counts use the full meshes (ghost layers included).
 """
 from __future__ import annotations
@@ -120,19 +125,24 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
             "wall_s": wall}
 
 
-def batched_adjoint(fvm, count=32, steps=300, slots=24, seed=77):
+def batched_adjoint(fvm, count=32, steps=300, slots=24, seed=77, recipe="fourier"):
     import importlib
     adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
-    rng = fvm.problems.SplitMix64(seed)
-    shapes = [(rng.randint(256, 768), rng.randint(256, 768)) for _ in range(count)]
-    Cs = [_cons(fvm.problems.fourier_perturbed(nx, ny, seed=i + 1)) for i, (ny, nx) in enumerate(shapes)]
+    if recipe == "riemann":
+        Us = fvm.problems.batched_instances(count=count, seed=seed, lo=256, hi=768)
+        shapes = [U.shape[:2] for U in Us]
+        Cs = [_cons(U) for U in Us]
+    else:
+        rng = fvm.problems.SplitMix64(seed)
+        shapes = [(rng.randint(256, 768), rng.randint(256, 768)) for _ in range(count)]
+        Cs = [_cons(fvm.problems.fourier_perturbed(nx, ny, seed=i + 1)) for i, (ny, nx) in enumerate(shapes)]
     adj = adjmod.Adjoint(shapes, [{}] * count, nsteps=steps, checkpoint_slots=slots)
     UNs = adj.forward(Cs)
     adj.backward([np.ones_like(u) for u in UNs])
     st = adj.stats()
     cells = sum(a * b for a, b in shapes)
     adj.close()
-    return {"instances": count, "steps": steps, "cells": cells, "checkpoint_slots": slots,
+    return {"recipe": recipe, "instances": count, "steps": steps, "cells": cells, "checkpoint_slots": slots,
             "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
             "forward_cell_updates_per_s": cells * steps / (st["forward_ms"] / 1000.0),
             "reverse_over_forward_step": st["reverse_step_ms"] / st["forward_step_ms"],
@@ -155,3 +165,27 @@ def peak(fvm, n=8192, steps=20, warmup=3):
     return {"mesh": [n, n], "steps": steps, "ms": ms, "cell_updates_per_s": rate,
             "launch": list(launch), "kernel_avg_ms": k_avg,
             "roofline_frac": BYTES_PER_CELL_UPDATE * n * n / (k_avg / 1000.0) / 1e9 / HBM_PEAK_GBS}
+
+
+# The reference's published wall-clock runs (four-quadrant data, CFL 0.5,
+# outflow): the demo main of test2D.cu:137-144 (2000^2, alpha 2.0, t_end
+# 0.8; README.md:2 "finishing in 15 minutes" on an RTX 4060, with a PNG
+# frame every 18 steps) and the figure's panels (a) (1000^2, alpha 1.9,
+# t_end 0.35: 1 min 18 s) and (c) (2000^2, alpha 1.7, t_end 0.35: 10 min
+# 44 s).  Frames are not written here: the time is the solve call's.
+PUBLISHED_RUNS = (("demo 2000^2 t_end 0.8 alpha 2.0", 2000, 0.8, 2.0, 900.0),
+                  ("figure (a) 1000^2 t_end 0.35 alpha 1.9", 1000, 0.35, 1.9, 78.0),
+                  ("figure (c) 2000^2 t_end 0.35 alpha 1.7", 2000, 0.35, 1.7, 644.0))
+
+
+def published(fvm):
+    out = []
+    for name, n, t_end, alpha, ref_s in PUBLISHED_RUNS:
+        U0 = fvm.problems.initialize_4shocks(n)
+        t0 = time.perf_counter()
+        _, nsteps, ms = fvm.solve(U0, t_end=t_end, cfl=0.5, alpha=alpha, gamma=1.4, strategy=0)
+        wall = time.perf_counter() - t0
+        out.append({"run": name, "mesh": [n, n], "steps": nsteps, "loop_s": ms / 1000.0,
+                    "call_wall_s": wall, "cell_updates_per_s": n * n * nsteps / (ms / 1000.0),
+                    "published_wall_s": ref_s, "published_over_call_wall": ref_s / wall})
+    return out

@@ -70,8 +70,9 @@ def main():
     ap.add_argument("--launch", default="",
                     help="strip_rows,waves_per_block,persistent: fixed geometry instead of autotuning")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched, adjoint, batched-adjoint and 8192^2 legs")
-    ap.add_argument("--leg", choices=["dropin", "batched", "adjoint", "batched_adjoint", "peak_8192"],
+                    help="skip the drop-in, batched, adjoint, batched-adjoint, 8192^2 and published-run legs")
+    ap.add_argument("--leg", choices=["dropin", "batched", "adjoint", "batched_adjoint",
+                                      "batched_adjoint_riemann", "peak_8192", "published"],
                     help="run only this extra leg and print its record")
     ap.add_argument("--adj-fd-cells", type=int, default=50,
                     help="cells of the adjoint leg's central-difference check (0 skips it)")
@@ -109,7 +110,9 @@ def main():
         fn = {"dropin": lambda f: extras.dropin(f, args.n, args.steps or -1, fvm.MODE_FAST),
               "batched": extras.batched,
               "adjoint": lambda f: extras.adjoint(f, fd_cells=args.adj_fd_cells),
-              "batched_adjoint": extras.batched_adjoint, "peak_8192": extras.peak}[args.leg]
+              "batched_adjoint": extras.batched_adjoint,
+              "batched_adjoint_riemann": lambda f: extras.batched_adjoint(f, recipe="riemann"),
+              "peak_8192": extras.peak, "published": extras.published}[args.leg]
         print(json.dumps({args.leg: fn(fvm)}))
         return
 
@@ -198,7 +201,9 @@ def main():
         legs = (("dropin", lambda f: extras.dropin(f, N, steps, mode, out["ms_per_step"])),
                 ("batched", extras.batched),
                 ("adjoint", lambda f: extras.adjoint(f, fd_cells=args.adj_fd_cells)),
-                ("batched_adjoint", extras.batched_adjoint), ("peak_8192", extras.peak))
+                ("batched_adjoint", extras.batched_adjoint),
+                ("batched_adjoint_riemann", lambda f: extras.batched_adjoint(f, recipe="riemann")),
+                ("peak_8192", extras.peak), ("published", extras.published))
         for key, fn in legs:
             try:
                 out[key] = fn(fvm)
