@@ -208,16 +208,31 @@ struct Engine {
     // column windows still fits the fewest rounds of resident waves that
     // keep strips at most 96 rows high.  mesh_strips then cuts them so that
     // one or two rounds are filled exactly.  At 2000 x 2000 on 256 CUs this
-    // is one round at R = 34, the geometry the autotuner picks.
+    // is one round at R = 34, the geometry the autotuner picks.  A strip
+    // also pays a fixed cost (its halo rows and the march's start), so when
+    // one round would give strips below kMinOneMeshRows rows, half, a
+    // quarter, ... of a round of taller strips is used instead (measured
+    // per step: 1000^2 0.101 -> 0.078 ms at R = 18, 768^2 0.089 -> 0.058 ms
+    // at R = 10, 512^2 0.061 -> 0.045 ms at R = 10).
+    static constexpr long long kMinOneMeshRows = 10;
     int one_mesh_rows() const {
         const InstDesc &d = hd[0];
         const long long H = d.ny - 4;
         const long long slots = (long long)std::max(1, step_kernel_occupancy(exact, 4)) * 4 * cu_count;
+        auto rows_for = [&](long long per_window) { return (H + per_window - 1) / per_window; };
         for (long long k = 1; k <= 64; k++) {
             const long long per_window = k * slots / d.nw;
             if (per_window < 1) continue;
-            const long long r = (H + per_window - 1) / per_window;
-            if (r <= 96) return (int)std::max<long long>(4, r);
+            const long long r = rows_for(per_window);
+            if (r > 96) continue;
+            if (k == 1 && r < kMinOneMeshRows) {
+                for (long long div = 2; div <= 64; div *= 2) {
+                    const long long pw = std::max(1ll, slots / div / d.nw);
+                    const long long rr = rows_for(pw);
+                    if (rr >= kMinOneMeshRows || pw == 1) return (int)std::min(96ll, rr);
+                }
+            }
+            return (int)std::max<long long>(4, r);
         }
         return 64;
     }
