@@ -83,28 +83,37 @@ def dropin(fvm, n=2000, steps=-1, mode=0, resident_ms_per_step=None):
     return rec
 
 
-def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
-    """Misfit gradient through the adjoint C ABI (host-pointer entry points;
-    the torch.autograd wrapper over the same calls is exercised by the tests)."""
-    import importlib
-    adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
-    C0 = _cons(fvm.problems.fourier_perturbed(n, n, seed=1))
-    Ct = _cons(fvm.problems.fourier_perturbed(n, n, seed=2))
-    adj = adjmod.Adjoint([(n, n)], [{}], nsteps=steps, checkpoint_slots=slots)
-    target = adj.forward([Ct])[0][0]
-    t0 = time.time()
+def _velocity_modes(U5s, n, seed=123, modes=8, amp=0.02):
+    """Seeded random-phase modes (k ~ U{1..8}^2, amplitude ~ U[0, amp]) added
+    to u and v of each state in place: no limiter starts on its kink."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    for U in U5s:
+        for c in (2, 3):
+            for _ in range(modes):
+                kx, ky = rng.integers(1, 9, size=2)
+                U[..., c] += rng.uniform(0, amp) * np.sin(2 * np.pi * (kx * X + ky * Y) +
+                                                          rng.uniform(0, 2 * np.pi))
+
+
+def _misfit_gradient(adj, C0, target):
+    """L = 1/2 sum (rho_N - target)^2 and dL/dU_0."""
     UN = adj.forward([C0])[0]
     loss = 0.5 * float(((UN[0] - target) ** 2).sum())
     gN = np.zeros_like(UN)
     gN[0] = UN[0] - target
-    g = adj.backward([gN])[0]
-    wall = time.time() - t0
-    st = adj.stats()
-    rng = np.random.default_rng(11)
+    return loss, adj.backward([gN])[0]
+
+
+def _fd_devs(adj, C0, target, g, cells, h=1e-6, seed=11):
+    """Central differences of the misfit at ``cells`` seeded (component, cell)
+    pairs against the gradient, relative to max(|FD|, 1e-6 max|g|)."""
+    n = C0.shape[1]
+    rng = np.random.default_rng(seed)
     devs = []
-    for _ in range(fd_cells):
+    for _ in range(cells):
         k, j, i = rng.integers(0, 4), rng.integers(0, n), rng.integers(0, n)
-        h = 1e-6
         Cp, Cm = C0.copy(), C0.copy()
         Cp[k, j, i] += h
         Cm[k, j, i] -= h
@@ -112,8 +121,33 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
         b = adj.forward([Cm])[0][0]
         fd = float((0.5 * (a - b) * (a + b - 2 * target)).sum()) / (2 * h)
         devs.append(abs(fd - g[k, j, i]) / max(abs(fd), 1e-6 * np.abs(g).max()))
+    return np.array(devs) if devs else np.zeros(1)
+
+
+def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
+    """Misfit gradient through the adjoint C ABI (host-pointer entry points;
+    the torch.autograd wrapper over the same calls is exercised by the tests),
+    timed on the specified data; central differences at ``fd_cells`` seeded
+    cells on the specified data (u = v = 0.1 exactly: limiters on their
+    kinks) and on the same data with random-phase velocity modes added."""
+    import importlib
+    adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
+    C0 = _cons(fvm.problems.fourier_perturbed(n, n, seed=1))
+    Ct = _cons(fvm.problems.fourier_perturbed(n, n, seed=2))
+    adj = adjmod.Adjoint([(n, n)], [{}], nsteps=steps, checkpoint_slots=slots)
+    target = adj.forward([Ct])[0][0]
+    t0 = time.time()
+    loss, g = _misfit_gradient(adj, C0, target)
+    wall = time.time() - t0
+    st = adj.stats()
+    devs = _fd_devs(adj, C0, target, g, fd_cells)
+    U5s = [fvm.problems.fourier_perturbed(n, n, seed=1), fvm.problems.fourier_perturbed(n, n, seed=2)]
+    _velocity_modes(U5s, n)
+    C0v, Ctv = _cons(U5s[0]), _cons(U5s[1])
+    target_v = adj.forward([Ctv])[0][0]
+    _, g_v = _misfit_gradient(adj, C0v, target_v)
+    devs_v = _fd_devs(adj, C0v, target_v, g_v, fd_cells)
     adj.close()
-    devs = np.array(devs) if devs else np.zeros(1)
     return {"mesh": [n, n], "steps": steps, "checkpoint_slots": slots, "loss": loss,
             "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
             "forward_step_ms": st["forward_step_ms"], "reverse_step_ms": st["reverse_step_ms"],
@@ -122,7 +156,9 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
             "recompute_steps": st["recompute_steps"], "schedule_cost": st["schedule_cost"],
             "fd_cells": fd_cells, "fd_h": 1e-6, "fd_rel_dev_median": float(np.median(devs)),
             "fd_rel_dev_p90": float(np.percentile(devs, 90)), "fd_rel_dev_max": float(devs.max()),
-            "wall_s": wall}
+            "tie_free_fd_rel_dev_median": float(np.median(devs_v)),
+            "tie_free_fd_rel_dev_p90": float(np.percentile(devs_v, 90)),
+            "tie_free_fd_rel_dev_max": float(devs_v.max()), "wall_s": wall}
 
 
 def batched_adjoint(fvm, count=32, steps=300, slots=24, seed=77, recipe="fourier"):
