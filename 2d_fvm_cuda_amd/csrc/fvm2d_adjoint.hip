@@ -207,10 +207,18 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
         if (!m.ydir) {
             // the X tasks cover every cell of U once: the first cell whose
             // wave speed is s_max (the forward loop's maximum over all cells,
-            // by the same function) is the arg-max
+            // by the same function) is the arg-max.  Regions of one constant
+            // state hold many such cells: only the wave's first (its lowest
+            // index) competes, and only while it is below the minimum found
+            // so far, so that the atomics do not queue on the one address.
             const unsigned long long sb = bits_of(d.fctl->smax);
-            if (own && bits_of(speed<false>(q.r, q.p, q.un, q.ut, d.gas)) == sb)
-                atomicMin(&d.ctl->argmax, (long long)t.w * d.nx + m.k);
+            const bool hit = own && bits_of(speed<false>(q.r, q.p, q.un, q.ut, d.gas)) == sb;
+            const unsigned long long hits = __ballot(hit);
+            if (hits && lane == __ffsll(hits) - 1) {
+                const long long idx = (long long)t.w * d.nx + m.k;
+                if (idx < __hip_atomic_load(&d.ctl->argmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMin(&d.ctl->argmax, idx);
+            }
             if (lane == 0 && !(sb <= 0x7ff0000000000000ull)) atomicOr(&d.ctl->status, 1);
         }
         if (!more) break;
