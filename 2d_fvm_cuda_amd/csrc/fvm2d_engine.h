@@ -303,10 +303,13 @@ struct Engine {
                     tasks.push_back(make_int4(i, w, bound(w, s), bound(w, s + 1)));
                 }
         } else {
+            // ns strips of at most R rows, heights differing by at most one
+            // (not ns - 1 strips of R rows and a short last one)
             for (int s = 0; s < ns; s++)
                 for (int w = 0; w < nw; w++) {
-                    const int r0 = 2 + s * R;
-                    tasks.push_back(make_int4(i, w, r0, std::min(ny - 2, r0 + R)));
+                    const int r0 = 2 + (int)((long long)s * H / ns);
+                    const int r1 = 2 + (int)((long long)(s + 1) * H / ns);
+                    tasks.push_back(make_int4(i, w, r0, r1));
                 }
         }
     }
