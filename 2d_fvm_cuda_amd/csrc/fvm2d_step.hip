@@ -156,7 +156,7 @@ __device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_
 // [r0, r1); local is the task's index among its instance's tasks.
 template <bool Exact>
 __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, int local,
-                          int lane) {
+                          int lane, const Xch &xa, const Xch &xb) {
     const int nx = d.nx, ny = d.ny;
     const long long P = d.plane;
     const double t = ctl->t, dt = ctl->dt;
@@ -192,17 +192,23 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
 
     unsigned long long mx = 0ull;
     Q u_n = load(r0 + 2);
+    xput(xa, u_n);
     for (int j = r0; j < r1; j++) {
         const Q u_cur = u_n;
+        const Q uL = xget(xa, xa.w), uR = xget(xa, xa.e);
         if (j + 1 < r1) u_n = load(j + 3);  // prefetch the next row
-        // branch B, first sweep: B1 row j
-        const Q b1 = ymarch_step<Exact>(mu, swap_uv(u_cur), LY, G, dt, lamy);
+        // branch B, first sweep: B1 row j (exchanged at once, needed after A)
+        const Q b1 = swap_uv(ymarch_step<Exact>(mu, swap_uv(u_cur), LY, G, dt, lamy));
+        xput(xb, b1);
         // branch A, first sweep on the look-ahead row: A1 row j+2
-        const Q a1 = xsweep<Exact>(u_cur, LX, G, dt, lamx);
+        const Q a1 = xsweep_pre<Exact>(u_cur, uL, uR, LX, G, dt, lamx);
         // branch A, second sweep: A row j
         const Q a = swap_uv(ymarch_step<Exact>(ma, swap_uv(a1), LY, G, dt, lamy));
         // branch B, second sweep: B row j
-        const Q b = xsweep<Exact>(swap_uv(b1), LX, G, dt, lamx);
+        const Q bL = xget(xb, xb.w), bR = xget(xb, xb.e);
+        const Q b = xsweep_pre<Exact>(b1, bL, bR, LX, G, dt, lamx);
+        // the next row's exchange, an iteration ahead of its use
+        if (j + 1 < r1) xput(xa, u_n);
         // src/basicFuns.cuh:264-274 average on primitives
         const double r = (a.r + b.r) * 0.5, p = (a.p + b.p) * 0.5;
         const double u = (a.un + b.un) * 0.5, v = (a.ut + b.ut) * 0.5;
@@ -262,6 +268,9 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int wpb = blockDim.x / kWave;
+    // per wave: the current input row's and the B1 row's exchange buffers
+    __shared__ double xbuf[4][2][4 * kWave];
+    const Xch xa = make_xch(xbuf[wid][0], lane), xb = make_xch(xbuf[wid][1], lane);
     int tk;
     if (persistent) {
         unsigned int v = 0;
@@ -279,7 +288,7 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
         Ctl *c = ctls + inst;
         const InstDesc &d = descs[inst];
         // the tasks of an instance hold positions task_base .. task_base + ntask - 1
-        if (!c->done) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane);
+        if (!c->done) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane, xa, xb);
         if (!persistent) break;
         unsigned int v = 0;
         if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
