@@ -141,12 +141,14 @@ def adjoint(fvm, n=512, steps=400, slots=32, fd_cells=50):
     wall = time.time() - t0
     st = adj.stats()
     devs = _fd_devs(adj, C0, target, g, fd_cells)
-    U5s = [fvm.problems.fourier_perturbed(n, n, seed=1), fvm.problems.fourier_perturbed(n, n, seed=2)]
-    _velocity_modes(U5s, n)
-    C0v, Ctv = _cons(U5s[0]), _cons(U5s[1])
-    target_v = adj.forward([Ctv])[0][0]
-    _, g_v = _misfit_gradient(adj, C0v, target_v)
-    devs_v = _fd_devs(adj, C0v, target_v, g_v, fd_cells)
+    devs_v = np.zeros(1)
+    if fd_cells > 0:
+        U5s = [fvm.problems.fourier_perturbed(n, n, seed=1), fvm.problems.fourier_perturbed(n, n, seed=2)]
+        _velocity_modes(U5s, n)
+        C0v, Ctv = _cons(U5s[0]), _cons(U5s[1])
+        target_v = adj.forward([Ctv])[0][0]
+        _, g_v = _misfit_gradient(adj, C0v, target_v)
+        devs_v = _fd_devs(adj, C0v, target_v, g_v, fd_cells)
     adj.close()
     return {"mesh": [n, n], "steps": steps, "checkpoint_slots": slots, "loss": loss,
             "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],

@@ -166,6 +166,13 @@ __device__ __forceinline__ int xcd_block() {
     return (b % kXcds) * (n / kXcds) + b / kXcds;
 }
 
+// A value of the forward loop's control block (constant while the reverse
+// step runs), read through the scalar cache: a vector load would make the
+// task wait on the vector memory counter, behind its look-ahead loads.
+__device__ __forceinline__ double ctl_value(const double *p) {
+    return *(const __attribute__((address_space(4))) double *)p;
+}
+
 // The task a wave works on is wave-uniform: keep it in scalar registers.
 __device__ __forceinline__ int4 uniform_task(const int4 *__restrict__ tasks, int tk) {
     const int4 v = tasks[tk];
@@ -184,22 +191,22 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
     int tk = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
     if (tk >= ntasks) return;
     int4 t = uniform_task(tasks, tk);
-    LineMap m = line_map(descs[t.x], t, lane, kFwdHalo, kFwdOwn);
-    Q q = load_line(descs[t.x].U, descs[t.x].plane, m);
+    Q q = load_line(descs[t.x].U, descs[t.x].plane, line_map(descs[t.x], t, lane, kFwdHalo, kFwdOwn));
     for (;;) {
         const int tn = tk + nwv;
         const bool more = tn < ntasks;
         int4 t2 = t;
-        LineMap m2 = m;
         Q q2 = q;
         if (more) {
             t2 = uniform_task(tasks, tn);
-            m2 = line_map(descs[t2.x], t2, lane, kFwdHalo, kFwdOwn);
-            q2 = load_line(descs[t2.x].U, descs[t2.x].plane, m2);
+            q2 = load_line(descs[t2.x].U, descs[t2.x].plane, line_map(descs[t2.x], t2, lane, kFwdHalo, kFwdOwn));
         }
+        // the line geometry is recomputed from the task, not carried over
+        // (a carried struct copy went through memory)
+        const LineMap m = line_map(descs[t.x], t, lane, kFwdHalo, kFwdOwn);
         const AdjDesc &d = descs[t.x];
         const Line L = line_of(d, m);
-        const double dt = d.fctl->dt;
+        const double dt = ctl_value(&d.fctl->dt);
         const Q qo = xsweep<false>(q, L, d.gas, dt, dt / L.h);
         const bool own = lane >= kFwdHalo && lane < kWave - kFwdHalo && m.k < m.n;
         const bool upd = m.k >= 2 && m.k <= m.n - 3;
@@ -211,12 +218,14 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
             // state hold many such cells: only the wave's first (its lowest
             // index) competes, and only while it is below the minimum found
             // so far, so that the atomics do not queue on the one address.
-            const unsigned long long sb = bits_of(d.fctl->smax);
+            const unsigned long long sb = bits_of(ctl_value(&d.fctl->smax));
             const bool hit = own && bits_of(speed<false>(q.r, q.p, q.un, q.ut, d.gas)) == sb;
+            const long long idx = (long long)t.w * d.nx + m.k;
             const unsigned long long hits = __ballot(hit);
-            if (hits && lane == __ffsll(hits) - 1) {
-                const long long idx = (long long)t.w * d.nx + m.k;
-                if (idx < __hip_atomic_load(&d.ctl->argmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            if (hits != 0ull) {
+                const int first = (int)__builtin_ctzll(hits);
+                if (lane == first &&
+                    idx < __hip_atomic_load(&d.ctl->argmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     atomicMin(&d.ctl->argmax, idx);
             }
             if (lane == 0 && !(sb <= 0x7ff0000000000000ull)) atomicOr(&d.ctl->status, 1);
@@ -224,7 +233,6 @@ __global__ void __launch_bounds__(256) adj_fwd_kernel(AdjDesc *__restrict__ desc
         if (!more) break;
         tk = tn;
         t = t2;
-        m = m2;
         q = q2;
     }
 }
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
     const int col = t.y * kWave + lane;
     const int colc = col < nx ? col : nx - 1;
     const bool valid = col < nx;
-    const double dt = d.fctl->dt;
+    const double dt = ctl_value(&d.fctl->dt);
     const Line LY = d.ly;
     const double lamy = dt / LY.h;
     auto load = [&](int j) -> Q {
@@ -295,7 +303,7 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
 __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, int lane, int mode, int part) {
     const Line L = line_of(d, m);
     const Gas &G = d.gas;
-    const double dt = d.fctl->dt;
+    const double dt = ctl_value(&d.fctl->dt);
     const double lam = dt / L.h;
     const long long P = d.plane;
     const double *src = mode == 0 ? (m.ydir ? d.A1 : d.B1) : d.U;
