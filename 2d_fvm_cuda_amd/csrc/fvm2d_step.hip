@@ -151,6 +151,15 @@ __device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_
     c->waves_done = 0u;
 }
 
+// A control-block field read through the scalar cache.  The fields a task
+// reads (t, dt, steps, done) change only when the instance's last task
+// advances the block, after every other task of the launch has read them,
+// and each launch starts with the scalar cache invalidated.
+template <typename T>
+__device__ __forceinline__ T ctl_field(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 // ---------------------------------------------------------------- the step
 // One task: window w (columns w * kOwn .. + kOwn - 1 owned), output rows
 // [r0, r1); local is the task's index among its instance's tasks.
@@ -159,8 +168,8 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
                           int lane, const Xch &xa, const Xch &xb) {
     const int nx = d.nx, ny = d.ny;
     const long long P = d.plane;
-    const double t = ctl->t, dt = ctl->dt;
-    const int steps = ctl->steps;
+    const double t = ctl_field(&ctl->t), dt = ctl_field(&ctl->dt);
+    const int steps = ctl_field(&ctl->steps);
     const double *__restrict__ src = (steps & 1) ? d.buf1 : d.buf0;
     double *__restrict__ dst = (steps & 1) ? d.buf0 : d.buf1;
     const Gas G = d.gas;
@@ -288,7 +297,7 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
         Ctl *c = ctls + inst;
         const InstDesc &d = descs[inst];
         // the tasks of an instance hold positions task_base .. task_base + ntask - 1
-        if (!c->done) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane, xa, xb);
+        if (!ctl_field(&c->done)) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane, xa, xb);
         if (!persistent) break;
         unsigned int v = 0;
         if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
