@@ -36,12 +36,20 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
     return (unsigned long long)__double_as_longlong(x);
 }
 
+// State planes live in global memory: accessed through global-address-space
+// pointers the loads and stores are global_* instructions, which count only
+// on the vector memory counter (flat ones count on the LDS/scalar counter
+// too, so every wait for an LDS exchange would also wait for the row loads
+// in flight).
+using gdouble = __attribute__((address_space(1))) double;
+
 __device__ __forceinline__ void store4(double *base, long long plane, long long off, double r,
                                        double p, double u, double v) {
-    base[off] = r;
-    base[plane + off] = p;
-    base[2 * plane + off] = u;
-    base[3 * plane + off] = v;
+    gdouble *g = (gdouble *)base;
+    g[off] = r;
+    g[plane + off] = p;
+    g[2 * plane + off] = u;
+    g[3 * plane + off] = v;
 }
 
 // Write every ghost cell among the candidates whose fill source (or, for
@@ -183,7 +191,8 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
 
     auto load = [&](int j) -> Q {  // X-order
         const long long o = (long long)j * d.pitch + colc;
-        return Q{src[o], src[P + o], src[2 * P + o], src[3 * P + o]};
+        const gdouble *g = (const gdouble *)src;
+        return Q{g[o], g[P + o], g[2 * P + o], g[3 * P + o]};
     };
 
     // prologue: rows r0-2 .. r0+1
