@@ -35,6 +35,14 @@ constexpr int kAdjOwn = kWave - 2 * kAdjHalo;  // 56
 constexpr int kFwdHalo = 2;
 constexpr int kFwdOwn = kWave - 2 * kFwdHalo;  // 60
 
+// State and adjoint planes live in global memory: accessed through
+// global-address-space pointers the loads and stores are global_*
+// instructions, which count only on the vector memory counter (flat ones
+// also count on the counter that DPP/LDS and scalar-load waits use).
+using gdouble = __attribute__((address_space(1))) double;
+__device__ __forceinline__ const gdouble *gp(const double *a) { return (const gdouble *)a; }
+__device__ __forceinline__ gdouble *gp(double *a) { return (gdouble *)a; }
+
 // Geometry of one lane line of a task.
 struct LineMap {
     long long base;    // offset of line index 0
@@ -65,15 +73,17 @@ __device__ __forceinline__ LineMap line_map(const AdjDesc &d, const int4 &t, int
 
 __device__ __forceinline__ long long off_of(const LineMap &m) { return m.base + m.kc * m.stride; }
 
-__device__ __forceinline__ Q load_line(const double *a, long long plane, const LineMap &m) {
+__device__ __forceinline__ Q load_line(const double *ap, long long plane, const LineMap &m) {
     const long long o = off_of(m);
+    const gdouble *a = gp(ap);
     const Q q{a[o], a[plane + o], a[2 * plane + o], a[3 * plane + o]};
     return pick(m.ydir, swap_uv(q), q);
 }
 
-__device__ __forceinline__ void store_line(double *a, long long plane, const LineMap &m, const Q &q,
+__device__ __forceinline__ void store_line(double *ap, long long plane, const LineMap &m, const Q &q,
                                            bool accumulate) {
     const long long o = off_of(m);
+    gdouble *a = gp(ap);
     const Q x = pick(m.ydir, swap_uv(q), q);
     if (accumulate) {
         a[o] += x.r;
@@ -120,15 +130,17 @@ __device__ __forceinline__ bool is_interior(const AdjDesc &d, int j, int c) {
     return !(j < 2 || j > d.ny - 3 || c < 2 || c > d.nx - 3);
 }
 
-__device__ __forceinline__ Q lam_at(const AdjDesc &d, const double *lam, int j, int c) {
+__device__ __forceinline__ Q lam_at(const AdjDesc &d, const double *lamp, int j, int c) {
     const long long P = d.plane, o = (long long)j * d.pitch + c;
+    const gdouble *lam = gp(lamp);
     const Q s{lam[o], lam[P + o], lam[2 * P + o], lam[3 * P + o]};
     const bool in = is_interior(d, j, c);
     return Q{in ? s.r : 0.0, in ? s.p : 0.0, in ? s.un : 0.0, in ? s.ut : 0.0};
 }
 
-__device__ Q lamC_ghosts(const AdjDesc &d, const double *lam, int j, int c, Q s) {
+__device__ Q lamC_ghosts(const AdjDesc &d, const double *lamp, int j, int c, Q s) {
     const int nx = d.nx, ny = d.ny;
+    const gdouble *lam = gp(lamp);
     if (!is_interior(d, j, c)) return s;
     if (j == 2 || j == ny - 4 || j == ny - 3 || c == 2 || c == nx - 3) {
         const long long P = d.plane;
@@ -253,8 +265,8 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
     const AdjDesc &d = descs[t.x];
     const int nx = d.nx, ny = d.ny, r0 = t.z, r1 = t.w;
     const long long P = d.plane, pitch = d.pitch;
-    const double *__restrict__ U = d.U;
-    double *__restrict__ B1 = d.B1;
+    const gdouble *__restrict__ U = gp(d.U);
+    gdouble *__restrict__ B1 = gp(d.B1);
     const int col = t.y * kWave + lane;
     const int colc = col < nx ? col : nx - 1;
     const bool valid = col < nx;
@@ -384,7 +396,7 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, i
     double *out = mode == 0 ? (m.ydir ? d.lamA1 : d.lamB1) : d.lam_out;
     if (own) store_line(out, P, m, qb, false);
     const double part_sum = wave_sum(own ? dtb : 0.0);
-    if (lane == 0) d.dtpart[part] = part_sum;
+    if (lane == 0) gp(d.dtpart)[part] = part_sum;
 }
 
 // Each wave walks a strided list of tasks.  Two waves per SIMD: nothing is
