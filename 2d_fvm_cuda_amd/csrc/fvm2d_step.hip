@@ -214,7 +214,10 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
     for (int j = r0; j < r1; j++) {
         const Q u_cur = u_n;
         const Q uL = xget(xa, xa.w), uR = xget(xa, xa.e);
-        if (j + 1 < r1) u_n = load(j + 3);  // prefetch the next row
+        // prefetch the next row (unconditionally, clamped into the mesh: a
+        // load on only some paths leaves the compiler waiting for it, and
+        // for this row's stores, at the end of every iteration)
+        u_n = load(j + 3 < ny ? j + 3 : ny - 1);
         // branch B, first sweep: B1 row j (exchanged at once, needed after A)
         const Q b1 = swap_uv(ymarch_step<Exact>(mu, swap_uv(u_cur), LY, G, dt, lamy));
         xput(xb, b1);
@@ -226,7 +229,7 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
         const Q bL = xget(xb, xb.w), bR = xget(xb, xb.e);
         const Q b = xsweep_pre<Exact>(b1, bL, bR, LX, G, dt, lamx);
         // the next row's exchange, an iteration ahead of its use
-        if (j + 1 < r1) xput(xa, u_n);
+        xput(xa, u_n);
         // src/basicFuns.cuh:264-274 average on primitives
         const double r = (a.r + b.r) * 0.5, p = (a.p + b.p) * 0.5;
         const double u = (a.un + b.un) * 0.5, v = (a.ut + b.ut) * 0.5;
