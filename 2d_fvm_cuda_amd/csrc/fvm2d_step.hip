@@ -223,10 +223,11 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
         xput(xb, b1);
         // branch A, first sweep on the look-ahead row: A1 row j+2
         const Q a1 = xsweep_pre<Exact>(u_cur, uL, uR, LX, G, dt, lamx);
+        // B1's neighbours, read a Y-march step ahead of their use
+        const Q bL = xget(xb, xb.w), bR = xget(xb, xb.e);
         // branch A, second sweep: A row j
         const Q a = swap_uv(ymarch_step<Exact>(ma, swap_uv(a1), LY, G, dt, lamy));
         // branch B, second sweep: B row j
-        const Q bL = xget(xb, xb.w), bR = xget(xb, xb.e);
         const Q b = xsweep_pre<Exact>(b1, bL, bR, LX, G, dt, lamx);
         // the next row's exchange, an iteration ahead of its use
         xput(xa, u_n);
