@@ -12,13 +12,15 @@
 // powers of two), the linear-acoustics interface solution at u* = 0 with
 // unequal impedances, the continuous-data limit, and, on every branch, the
 // characteristic solution of the Lagrangian relaxation system wherever the
-// slopes leave the impedance unchanged (p rho_x + rho p_x = 0).  Parity
-// unpinned: only the coefficients of the star-region terms proportional to
-// the impedance slope c_x at u* != u_K (src/basicFuns.cuh:151-154, 165,
-// 168, 179, 182); they are the author's modelling choices (an exact
-// variable-impedance relaxation solution has other coefficients), so only a
-// compiled reference would fix them, and the restatement follows the source
-// token for token there.
+// slopes leave the impedance unchanged (p rho_x + rho p_x = 0).  No
+// mathematics fixes the coefficients of the star-region terms proportional
+// to the impedance slope c_x at u* != u_K (src/basicFuns.cuh:151-154, 165,
+// 168, 179, 182): they are the author's modelling choices (an exact
+// variable-impedance relaxation solution has other coefficients) and the
+// reference cannot be compiled here.  They are pinned to the reference's
+// source instead: tests/test_oracle_source_identity.py compares grp below
+// with the reference's GRP kernel token for token wherever the reference
+// checkout is present.
 #include "fvm2d_oracle.h"
 
 #include <algorithm>
