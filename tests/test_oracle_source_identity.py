@@ -94,6 +94,23 @@ def test_oracle_grp_is_the_reference_source_token_for_token():
                     f"  reference: {r}\n  oracle:    {o}")
 
 
+def test_oracle_flux_is_the_reference_source_token_for_token():
+    """The time-accurate flux (src/basicFuns.cuh:14-34; its return
+    expression, braces of the aggregate as the oracle's constructor call)."""
+    if not os.path.exists(REF_SRC):
+        pytest.skip(f"reference checkout not present ({REF_SRC})")
+    ref = region(open(REF_SRC).read(), "__device__ vec5d flux(", "// MUSCL reconstruction")
+    orc = region(open(ORACLE_SRC).read(), "V5 flux(const V5 &U", "// basicFuns.cuh:49-68")
+
+    def expr(text):
+        t = tokens(text[text.index("return"):])
+        t = t[:t.index(";")]
+        return ["(" if x == "{" else ")" if x == "}" else x for x in t if x != "V5"]
+
+    assert len(expr(ref)) > 200
+    assert expr(orc) == expr(ref)
+
+
 @pytest.mark.parametrize("old,new", [
     ("+ 1.5 * (u - u_r) * c_rx", "+ 1.6 * (u - u_r) * c_rx"),      # rho_t coefficient
     ("+ 0.5 * (u - u_l) * c_lx", "+ 0.5 * (u - u_r) * c_lx"),      # k_l side mix-up
