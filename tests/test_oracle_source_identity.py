@@ -11,7 +11,7 @@ reference's source, so this test compares the oracle's `grp`
 (oracle/fvm2d_oracle.cpp) with the body of the reference's GRP kernel
 `rel_RP_posi_fix` (src/basicFuns.cuh:86-189) token for token, from the
 slope preparation (e_lx) to the end of the star region.  The only
-differences allowed are spelled out in NORMALISE (std::sqrt for sqrt, the
+differences allowed are spelled out in normalise() (std::sqrt for sqrt, the
 oracle's tmax for thrust's max, and the oracle's branch index).  It reads
 the reference checkout, so it runs where one is present (FVM2D_REFERENCE_DIR,
 default /root/reference) and is skipped elsewhere.
@@ -118,7 +118,7 @@ def test_oracle_flux_is_the_reference_source_token_for_token():
     ("c_l / rho_l * (u - u_l) * c_lx", "c_l / rho_l * (u - u_l) * c_rx"),          # d_l slope
 ])
 def test_a_planted_mistake_is_caught(old, new):
-    """The comparison has teeth: each single-token mistake in the unpinned
+    """The comparison has teeth: each single-token mistake in the
     c_x terms, planted in a copy of the oracle's text, fails it."""
     ref = reference_grp_tokens()
     src = open(ORACLE_SRC).read()
