@@ -52,10 +52,16 @@ def _cases(problems):
     cases.append(("alpha2_4shocks96", problems.initialize_4shocks(96), dict(t_end=0.05, alpha=2.0)))
     cases.append(("gamma11_2mach24", problems.initialize_2Mach(24),
                   dict(t_end=0.01, max_steps=10, strategy=4, width=4.0, height=1.0, gamma=1.1)))
+    # extreme aspect ratios: two interior columns over many strips, and three
+    # interior rows over many windows
+    cases.append(("rand6x3001", problems.random_four_quadrant(rng, 6, 3001),
+                  dict(t_end=1.0, max_steps=7, width=0.01, height=5.0)))
+    cases.append(("rand2999x7", problems.random_four_quadrant(rng, 2999, 7),
+                  dict(t_end=1.0, max_steps=7, width=5.0, height=0.01)))
     return cases
 
 
-NCASES = 15
+NCASES = 17
 
 
 @pytest.fixture(scope="module")

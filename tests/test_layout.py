@@ -26,6 +26,9 @@ def _check_cover(L, nx, ny):
 
 CASES = [(nx, ny, R) for nx in (64, 133, 517, 1000, 2000, 2900) for ny in (37, 256, 999, 2000, 3100)
          for R in (1, 4, 16, 33, 34, 64)]
+# extreme aspect ratios: one window of two interior columns, a few interior
+# rows over many windows, strips taller than the mesh
+CASES += [(nx, ny, R) for nx in (6, 7, 64, 2999) for ny in (6, 7, 3001) for R in (1, 3, 34, 96)]
 
 
 @pytest.mark.parametrize("wpb,pers", [(4, 0), (2, 0), (4, 1), (1, 0)])
