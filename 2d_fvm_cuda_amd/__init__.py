@@ -510,7 +510,7 @@ class RGRP_2D_CUDA:
         state as the reference copies it out, after the branch average and
         before that step's ghost fill (fvm2d_download_frame)."""
         import time
-        from .io import write_frame_pgm
+        from .io import write_frame_png
         gap = max(1, self._gap)
         s = Solver(self._Nx, self._Ny, width=self._width, height=self._height,
                    t_end=self._endTime, cfl=self._CFL, alpha=self._alpha, gamma=self._gamma,
@@ -526,7 +526,7 @@ class RGRP_2D_CUDA:
                 if steps < due:
                     break
                 done_steps = steps
-                write_frame_pgm(self._recordDir, s.download_frame()[..., 0], due - 1)
+                write_frame_png(self._recordDir, s.download_frame()[..., 0], due - 1)
                 due += gap
             ms = (time.perf_counter() - t0) * 1e3
             U = s.download()

@@ -9,8 +9,12 @@ mesh row (row 0 first).
 from __future__ import annotations
 
 import os
+import struct
+import zlib
 
 import numpy as np
+
+from .viridis import VIRIDIS_U8
 
 
 def write_matrix_txt(dir: str, U: np.ndarray, cTime: float) -> None:
@@ -38,28 +42,60 @@ def normalize_u8(a: np.ndarray) -> np.ndarray:
     return np.clip(np.rint(a * scale - lo * scale), 0, 255).astype(np.uint8)
 
 
-def write_frame_pgm(dir: str, rho: np.ndarray, index: int) -> str:
-    """Record-mode frame: ``<dir>/<index>.pgm``, the density of every cell
-    (ghost cells included, row 0 first) as an 8-bit binary PGM after
-    normalize_u8.  The reference writes ``<index>.png`` through OpenCV with
-    the viridis colour map applied to the same 8-bit image
-    (src/wrappers.cuh:323-345); the grey levels here are that image before
-    colouring."""
+def colour_viridis(img: np.ndarray) -> np.ndarray:
+    """The 8-bit image through the viridis table (the cv::applyColorMap(...,
+    COLORMAP_VIRIDIS) step of src/wrappers.cuh:342): shape + (3,), RGB."""
+    lut = np.frombuffer(VIRIDIS_U8, dtype=np.uint8).reshape(256, 3)
+    return lut[np.asarray(img, dtype=np.uint8)]
+
+
+def _png_chunk(kind: bytes, data: bytes) -> bytes:
+    return (struct.pack(">I", len(data)) + kind + data +
+            struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF))
+
+
+def write_frame_png(dir: str, rho: np.ndarray, index: int) -> str:
+    """Record-mode frame ``<dir>/<index>.png``: the density of every cell
+    (ghost cells included, row 0 first) after normalize_u8, coloured with
+    viridis, as an 8-bit RGB PNG -- the file src/wrappers.cuh:323-345 writes
+    through OpenCV."""
     os.makedirs(dir, exist_ok=True)
-    img = normalize_u8(rho)
-    path = os.path.join(dir, f"{index}.pgm")
+    rgb = colour_viridis(normalize_u8(rho))
+    h, w = rgb.shape[:2]
+    raw = b"".join(b"\x00" + rgb[j].tobytes() for j in range(h))  # filter type 0
+    png = (b"\x89PNG\r\n\x1a\n" +
+           _png_chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
+           _png_chunk(b"IDAT", zlib.compress(raw)) + _png_chunk(b"IEND", b""))
+    path = os.path.join(dir, f"{index}.png")
     with open(path, "wb") as f:
-        f.write(b"P5\n%d %d\n255\n" % (img.shape[1], img.shape[0]))
-        f.write(np.ascontiguousarray(img).tobytes())
+        f.write(png)
     return path
 
 
-def read_pgm(path: str) -> np.ndarray:
-    """Read an 8-bit binary PGM written by write_frame_pgm."""
+def read_png(path: str) -> np.ndarray:
+    """Read an 8-bit RGB PNG without interlacing whose rows use filter type 0
+    (the frames written here and by include/fvm2d_frame.hpp); checks every
+    chunk's CRC.  Returns an (h, w, 3) uint8 array."""
     with open(path, "rb") as f:
         data = f.read()
-    parts = data.split(b"\n", 3)
-    if parts[0] != b"P5":
-        raise ValueError(f"{path}: not a binary PGM")
-    w, h = (int(x) for x in parts[1].split())
-    return np.frombuffer(parts[3], dtype=np.uint8, count=w * h).reshape(h, w)
+    if data[:8] != b"\x89PNG\r\n\x1a\n":
+        raise ValueError(f"{path}: not a PNG")
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        (crc,) = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        if crc != zlib.crc32(kind + body) & 0xFFFFFFFF:
+            raise ValueError(f"{path}: bad CRC in {kind!r}")
+        if kind == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, depth, ctype, _, _, interlace = hdr
+    if (depth, ctype, interlace) != (8, 2, 0):
+        raise ValueError(f"{path}: not an 8-bit RGB PNG without interlacing")
+    raw = np.frombuffer(zlib.decompress(idat), dtype=np.uint8).reshape(h, 3 * w + 1)
+    if np.any(raw[:, 0] != 0):
+        raise ValueError(f"{path}: rows with a filter other than type 0")
+    return raw[:, 1:].reshape(h, w, 3).copy()

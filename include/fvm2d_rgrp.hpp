@@ -10,10 +10,10 @@
 //
 // Record mode (setRecordModel, src/wrappers.cuh:147-152,253-261,323-345)
 // writes a frame of the density after the first step and every gapPerFrame
-// steps, named by the step index, as an 8-bit binary PGM <index>.pgm (the
-// min-max normalised image the reference colours with viridis and saves as
-// PNG through OpenCV, which this build does not use).  Like the reference's,
-// a frame holds the state before that step's ghost fill.
+// steps, named by the step index, as <index>.png: the min-max normalised
+// 8-bit image coloured with viridis, as the reference writes it through
+// OpenCV (include/fvm2d_frame.hpp writes the PNG without OpenCV).  Like the
+// reference's, a frame holds the state before that step's ghost fill.
 #pragma once
 
 #include <algorithm>
@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "fvm2d.h"
+#include "fvm2d_frame.hpp"
 
 struct vec5d {
     vec5d() : data{0.0, 0.0, 0.0, 0.0, 0.0} {}
@@ -148,7 +149,7 @@ class RGRP_2D_CUDA {
     // step and every gap_ further steps the state is copied out as the
     // reference copies it (after the branch average, before the ghost fill:
     // fvm2d_download_frame) and its density written as
-    // <recordDir_>/<step index>.pgm.
+    // <recordDir_>/<step index>.png.
     void solve_recording(const fvm2d_params &p, int *steps_out, double *ms_out) {
         fvm2d_solver *sv = nullptr;
         check(fvm2d_create(Nx_, Ny_, &p, &sv), "fvm2d_create");
@@ -183,7 +184,9 @@ class RGRP_2D_CUDA {
     }
 
     // Density min-max normalised to 0..255 and rounded (the
-    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340).
+    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340),
+    // coloured with viridis and written as <recordDir_>/<index>.png
+    // (src/wrappers.cuh:342-344).
     void write_frame(const std::vector<vec5d> &f, int index) const {
         double lo = f[0](0), hi = f[0](0);
         for (const auto &e : f) {
@@ -191,16 +194,12 @@ class RGRP_2D_CUDA {
             hi = std::max(hi, e(0));
         }
         const double scale = hi > lo ? 255.0 / (hi - lo) : 0.0;
-        std::ofstream out(recordDir_ + "/" + std::to_string(index) + ".pgm", std::ios::binary);
-        out << "P5\n" << Nx_ << " " << Ny_ << "\n255\n";
-        std::vector<unsigned char> row((size_t)Nx_);
-        for (int j = 0; j < Ny_; ++j) {
-            for (int i = 0; i < Nx_; ++i) {
-                const double v = std::nearbyint(f[(size_t)j * Nx_ + i](0) * scale - lo * scale);
-                row[i] = (unsigned char)std::min(255.0, std::max(0.0, v));
-            }
-            out.write(reinterpret_cast<const char *>(row.data()), (std::streamsize)row.size());
+        std::vector<unsigned char> gray((size_t)Nx_ * Ny_);
+        for (size_t k = 0; k < gray.size(); ++k) {
+            const double v = std::nearbyint(f[k](0) * scale - lo * scale);
+            gray[k] = (unsigned char)std::min(255.0, std::max(0.0, v));
         }
+        fvm2d::write_frame_png(recordDir_ + "/" + std::to_string(index) + ".png", Nx_, Ny_, gray.data());
     }
 
     std::vector<std::vector<vec5d>> vectorized() const {

@@ -22,14 +22,15 @@ io = importlib.import_module("2d_fvm_cuda_amd.io")
 def _caller():
     lib_dir = os.path.join(ROOT, "2d_fvm_cuda_amd")
     src = CALLER + ".cpp"
-    if not os.path.exists(CALLER) or os.path.getmtime(CALLER) < os.path.getmtime(src):
+    deps = [src] + glob.glob(os.path.join(ROOT, "include", "*.h*"))
+    if not os.path.exists(CALLER) or any(os.path.getmtime(CALLER) < os.path.getmtime(d) for d in deps):
         subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src,
                         "-L", lib_dir, "-lfvm2d", "-Wl,-rpath," + lib_dir, "-o", CALLER], check=True)
     return CALLER
 
 
 def _frames(d):
-    return sorted(int(os.path.basename(p)[:-4]) for p in glob.glob(os.path.join(d, "*.pgm")))
+    return sorted(int(os.path.basename(p)[:-4]) for p in glob.glob(os.path.join(d, "*.png")))
 
 
 @pytest.mark.parametrize("strategy", [0, 4])
@@ -70,8 +71,8 @@ def test_python_mirror_record_mode(gpu, orc, problems, tmp_path):
     assert np.array_equal(U[..., :4], ref[..., :4])
     assert _frames(str(tmp_path)) == idx == list(range(0, n, 4))
     for k, f in zip(idx, frames):
-        assert np.array_equal(io.read_pgm(os.path.join(str(tmp_path), f"{k}.pgm")),
-                              io.normalize_u8(f[..., 0])), k
+        assert np.array_equal(io.read_png(os.path.join(str(tmp_path), f"{k}.png")),
+                              io.colour_viridis(io.normalize_u8(f[..., 0]))), k
 
 
 def test_drop_in_caller_matches_oracle_records_and_writes_text(gpu, orc, problems, tmp_path):
@@ -101,4 +102,5 @@ def test_drop_in_caller_matches_oracle_records_and_writes_text(gpu, orc, problem
     subprocess.run([exe, "48", "0.02", "1", rec, "3"], capture_output=True, text=True, check=True)
     assert _frames(rec) == idx == list(range(0, n_ref, 3))
     for k, f in zip(idx, frames):
-        assert np.array_equal(io.read_pgm(os.path.join(rec, f"{k}.pgm")), io.normalize_u8(f[..., 0])), k
+        assert np.array_equal(io.read_png(os.path.join(rec, f"{k}.png")),
+                              io.colour_viridis(io.normalize_u8(f[..., 0]))), k

@@ -1,11 +1,18 @@
-"""Host-side checks of the record-mode frame writer and text output
-(2d_fvm_cuda_amd/io.py): normalisation, PGM layout, round trip."""
+"""Host-side checks of the record-mode frame writers and text output
+(2d_fvm_cuda_amd/io.py, include/fvm2d_frame.hpp): normalisation, the
+viridis table, PNG layout and round trip."""
 import importlib
 import os
+import re
+import struct
+import subprocess
 
 import numpy as np
+import pytest
 
 io = importlib.import_module("2d_fvm_cuda_amd.io")
+VIRIDIS_U8 = importlib.import_module("2d_fvm_cuda_amd.viridis").VIRIDIS_U8
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_normalize_u8_minmax_rounding():
@@ -17,17 +24,58 @@ def test_normalize_u8_minmax_rounding():
     assert io.normalize_u8(np.full((3, 4), 7.0)).max() == 0
 
 
-def test_frame_pgm_round_trip(tmp_path):
+def test_frame_png_round_trip(tmp_path):
     rng = np.random.default_rng(0)
     rho = rng.uniform(0.1, 2.0, size=(7, 11))
-    path = io.write_frame_pgm(str(tmp_path / "frames"), rho, 36)
-    assert os.path.basename(path) == "36.pgm"
-    head = open(path, "rb").read(12)
-    assert head.startswith(b"P5\n11 7\n255\n")
-    back = io.read_pgm(path)
-    assert back.shape == (7, 11)
-    assert np.array_equal(back, io.normalize_u8(rho))
-    assert back.min() == 0 and back.max() == 255
+    path = io.write_frame_png(str(tmp_path / "frames"), rho, 36)
+    assert os.path.basename(path) == "36.png"
+    head = open(path, "rb").read(24)
+    assert head[:8] == b"\x89PNG\r\n\x1a\n" and head[12:16] == b"IHDR"
+    assert struct.unpack(">II", head[16:24]) == (11, 7)
+    back = io.read_png(path)
+    assert back.shape == (7, 11, 3)
+    assert np.array_equal(back, io.colour_viridis(io.normalize_u8(rho)))
+    lut = np.frombuffer(VIRIDIS_U8, dtype=np.uint8).reshape(256, 3)
+    assert np.array_equal(back[io.normalize_u8(rho) == 0][0], lut[0])
+    assert np.array_equal(back[io.normalize_u8(rho) == 255][0], lut[255])
+
+
+def test_viridis_table():
+    """The Python and C++ copies of the table are the same, run from
+    (68, 1, 84) to (253, 231, 37), and are round(255 v) of the viridis data
+    where matplotlib (which ships that data) is importable."""
+    lut = np.frombuffer(VIRIDIS_U8, dtype=np.uint8).reshape(256, 3)
+    hpp = open(os.path.join(ROOT, "include", "fvm2d_viridis.hpp")).read()
+    body = hpp[hpp.index("{", hpp.index("kViridis")) + 1:hpp.index("};")]
+    assert [int(x) for x in re.findall(r"\d+", body)] == list(VIRIDIS_U8)
+    assert lut[0].tolist() == [68, 1, 84] and lut[255].tolist() == [253, 231, 37]
+    try:
+        from matplotlib import _cm_listed
+    except ImportError:
+        pytest.skip("matplotlib not importable")
+    ref = np.rint(255.0 * np.asarray(_cm_listed._viridis_data)).astype(np.uint8)
+    assert np.array_equal(lut, ref)
+
+
+@pytest.mark.parametrize("w,h", [(11, 7), (300, 100), (1, 1)])
+def test_cpp_frame_writer(tmp_path, w, h):
+    """include/fvm2d_frame.hpp (the drop-in header's frame writer, stored
+    deflate blocks: 300 x 100 needs two) decodes with zlib to the viridis
+    colouring of its input."""
+    exe = str(tmp_path / "frame_writer")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "frame_writer.cpp"), "-o", exe], check=True)
+    out = str(tmp_path / "f.png")
+    subprocess.run([exe, out, str(w), str(h)], check=True)
+    i, j = np.meshgrid(np.arange(w), np.arange(h))
+    gray = ((7 * i + 13 * j) % 256).astype(np.uint8)
+    assert np.array_equal(io.read_png(out), io.colour_viridis(gray))
+    try:
+        from PIL import Image
+    except ImportError:
+        return
+    # an independent PNG decoder reads the same pixels
+    assert np.array_equal(np.asarray(Image.open(out).convert("RGB")), io.colour_viridis(gray))
 
 
 def test_matrix_txt_names_and_format(tmp_path):
