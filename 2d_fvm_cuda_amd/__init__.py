@@ -508,27 +508,39 @@ class RGRP_2D_CUDA:
         named by the index of the step (0, gap, 2 gap, ...).  The loop runs
         on the device in chunks of gapPerFrame steps; each frame is the
         state as the reference copies it out, after the branch average and
-        before that step's ghost fill (fvm2d_download_frame)."""
+        before that step's ghost fill (fvm2d_download_frame).  Frames are
+        encoded and written by a pool of host threads while the device
+        steps on; like the reference's, the computation time includes
+        writing them (the call returns after the last one is written)."""
         import time
+        from concurrent.futures import ThreadPoolExecutor
         from .io import write_frame_png
         gap = max(1, self._gap)
+        workers = max(1, min(16, os.cpu_count() or 1))
         s = Solver(self._Nx, self._Ny, width=self._width, height=self._height,
                    t_end=self._endTime, cfl=self._CFL, alpha=self._alpha, gamma=self._gamma,
                    strategy=int(self._strategy), mode=self._mode)
         try:
             s.upload(self._U0)
-            t0 = time.perf_counter()
-            s.begin()
-            done_steps, due = 0, 1
-            while True:
-                s.advance(due - done_steps)
-                steps, _, _ = s.sync()
-                if steps < due:
-                    break
-                done_steps = steps
-                write_frame_png(self._recordDir, s.download_frame()[..., 0], due - 1)
-                due += gap
-            ms = (time.perf_counter() - t0) * 1e3
+            with ThreadPoolExecutor(max_workers=workers) as pool:
+                pending = []
+                t0 = time.perf_counter()
+                s.begin()
+                done_steps, due = 0, 1
+                while True:
+                    s.advance(due - done_steps)
+                    steps, _, _ = s.sync()
+                    if steps < due:
+                        break
+                    done_steps = steps
+                    rho = np.ascontiguousarray(s.download_frame()[..., 0])
+                    pending.append(pool.submit(write_frame_png, self._recordDir, rho, due - 1))
+                    if len(pending) >= 2 * workers:  # bound the frames held in memory
+                        pending.pop(0).result()
+                    due += gap
+                for f in pending:
+                    f.result()
+                ms = (time.perf_counter() - t0) * 1e3
             U = s.download()
         finally:
             s.close()

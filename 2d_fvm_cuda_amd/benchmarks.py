@@ -18,10 +18,17 @@ peak()             8192^2 plain forward (2.1 GB of state), 20 timed steps.
 published()        the runs whose wall-clock times the reference publishes
                    (README.md:2, test2D.cu:137-144, its figure's panels (a)
                    and (c)), solved by the drop-in entry point.
+published_record() the demo main as published, record mode included
+                   (test2D.cu:134-152: a viridis PNG frame every 18 steps,
+                   then writeFinalResultTo_txt), through the Python mirror
+                   of the reference's class; files in a temporary directory.
 Cell-update counts use the full meshes (ghost layers included).
 """
 from __future__ import annotations
 
+import os
+import shutil
+import tempfile
 import time
 
 import numpy as np
@@ -241,3 +248,33 @@ def published(fvm):
                     "call_wall_s": wall, "cell_updates_per_s": n * n * nsteps / (ms / 1000.0),
                     "published_wall_s": ref_s, "published_over_call_wall": ref_s / wall})
     return out
+
+
+def published_record(fvm):
+    """test2D.cu:134-152 as written: 2000^2 four-quadrant data, [0,1]^2,
+    t_end 0.8, CFL 0.5, outflow, alpha 2.0, gamma 1.4, a frame every 18
+    steps into ./pngs, then writeFinalResultTo_txt(".") -- here into a
+    temporary directory that is removed afterwards.  The published time
+    ("finishing in 15 minutes", README.md:2) is this program's."""
+    d = tempfile.mkdtemp(prefix="fvm2d_demo_")
+    try:
+        r = fvm.RGRP_2D_CUDA(fvm.problems.initialize_4shocks(2000))
+        r.setSpatialLayout(1.0, 1.0)
+        r.setTimeLayout(0.8, 0.5)
+        r.setGhostCellStrategy(fvm.ghostCellStrategy.outflow)
+        r.setAlpha(2.0)
+        r.setGamma(1.4)
+        pngs = os.path.join(d, "pngs")
+        r.setRecordModel(True, 18, pngs)
+        t0 = time.perf_counter()
+        r.solve()
+        r.writeFinalResultTo_txt(d)
+        wall = time.perf_counter() - t0
+        frames = sorted(os.listdir(pngs))
+        nbytes = sum(os.path.getsize(os.path.join(pngs, f)) for f in frames)
+        return {"run": "demo main 2000^2 t_end 0.8 alpha 2.0, PNG frame every 18 steps",
+                "steps": r.getTimeStagesCount(), "frames": len(frames), "frame_MB": nbytes / 1e6 / len(frames),
+                "computation_time_s": r.getComputationTime_ms() / 1000.0, "program_wall_s": wall,
+                "published_wall_s": 900.0, "published_over_program_wall": 900.0 / wall}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)

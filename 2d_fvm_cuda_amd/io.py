@@ -58,14 +58,14 @@ def write_frame_png(dir: str, rho: np.ndarray, index: int) -> str:
     """Record-mode frame ``<dir>/<index>.png``: the density of every cell
     (ghost cells included, row 0 first) after normalize_u8, coloured with
     viridis, as an 8-bit RGB PNG -- the file src/wrappers.cuh:323-345 writes
-    through OpenCV."""
+    through OpenCV (zlib level 1, OpenCV's default for PNG)."""
     os.makedirs(dir, exist_ok=True)
     rgb = colour_viridis(normalize_u8(rho))
     h, w = rgb.shape[:2]
     raw = b"".join(b"\x00" + rgb[j].tobytes() for j in range(h))  # filter type 0
     png = (b"\x89PNG\r\n\x1a\n" +
            _png_chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
-           _png_chunk(b"IDAT", zlib.compress(raw)) + _png_chunk(b"IEND", b""))
+           _png_chunk(b"IDAT", zlib.compress(raw, 1)) + _png_chunk(b"IEND", b""))
     path = os.path.join(dir, f"{index}.png")
     with open(path, "wb") as f:
         f.write(png)

@@ -72,7 +72,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the drop-in, batched, adjoint, batched-adjoint, 8192^2 and published-run legs")
     ap.add_argument("--leg", choices=["dropin", "batched", "adjoint", "batched_adjoint",
-                                      "batched_adjoint_riemann", "peak_8192", "published"],
+                                      "batched_adjoint_riemann", "peak_8192", "published",
+                                      "published_record"],
                     help="run only this extra leg and print its record")
     ap.add_argument("--adj-fd-cells", type=int, default=50,
                     help="cells of the adjoint leg's central-difference check (0 skips it)")
@@ -112,7 +113,8 @@ def main():
               "adjoint": lambda f: extras.adjoint(f, fd_cells=args.adj_fd_cells),
               "batched_adjoint": extras.batched_adjoint,
               "batched_adjoint_riemann": lambda f: extras.batched_adjoint(f, recipe="riemann"),
-              "peak_8192": extras.peak, "published": extras.published}[args.leg]
+              "peak_8192": extras.peak, "published": extras.published,
+              "published_record": extras.published_record}[args.leg]
         print(json.dumps({args.leg: fn(fvm)}))
         return
 
