@@ -2,13 +2,19 @@
 // density image coloured with viridis and written as an RGB PNG, the file
 // the reference's writeStagesTo_png produces through cv::applyColorMap and
 // cv::imwrite (src/wrappers.cuh:323-345).  The PNG holds one IDAT zlib
-// stream of stored (uncompressed) deflate blocks, filter type 0 on every
-// row (PNG specification, RFC 1950, RFC 1951).
+// stream, filter type 0 on every row (PNG specification, RFC 1950, RFC
+// 1951): compressed by zlib's compress2 at level 1 (OpenCV's default for
+// PNG) when the process can load libz.so.1 -- looked up at run time, so a
+// caller links nothing more -- else made of stored (uncompressed) deflate
+// blocks, which is also forced by FVM2D_PNG_STORED=1 in the environment.
 #pragma once
+
+#include <dlfcn.h>
 
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -48,6 +54,32 @@ inline void png_chunk(std::vector<unsigned char> &out, const char *type, const s
     png_put32(out, png_crc(out.data() + start, out.size() - start) ^ 0xffffffffu);
 }
 
+// zlib stream of raw by zlib's compress2 (level 1); false if libz cannot be
+// loaded or fails.
+inline bool png_zlib_compress(const std::vector<unsigned char> &raw, std::vector<unsigned char> &z) {
+    using compress2_fn = int (*)(unsigned char *, unsigned long *, const unsigned char *, unsigned long, int);
+    using bound_fn = unsigned long (*)(unsigned long);
+    struct Zlib {
+        compress2_fn compress2 = nullptr;
+        bound_fn bound = nullptr;
+        Zlib() {
+            const char *e = std::getenv("FVM2D_PNG_STORED");
+            if (e && *e && *e != '0') return;
+            if (void *h = dlopen("libz.so.1", RTLD_NOW | RTLD_LOCAL)) {
+                compress2 = reinterpret_cast<compress2_fn>(dlsym(h, "compress2"));
+                bound = reinterpret_cast<bound_fn>(dlsym(h, "compressBound"));
+            }
+        }
+    };
+    static const Zlib zl;
+    if (!zl.compress2 || !zl.bound) return false;
+    unsigned long n = zl.bound((unsigned long)raw.size());
+    z.resize(n);
+    if (zl.compress2(z.data(), &n, raw.data(), (unsigned long)raw.size(), 1) != 0) return false;
+    z.resize(n);
+    return true;
+}
+
 // An 8-bit RGB image of w x h pixels (rgb: 3 bytes per pixel, rows top to
 // bottom) as a PNG file; throws std::runtime_error if the file cannot be
 // written.
@@ -61,24 +93,27 @@ inline void write_png_rgb(const std::string &path, int w, int h, const unsigned 
         raw.push_back(0);
         raw.insert(raw.end(), rgb + j * row, rgb + (j + 1) * row);
     }
-    // zlib stream: header 0x78 0x01 (deflate, 32 KiB window), stored blocks
-    // of at most 65535 bytes, Adler-32 of the raw data
-    std::vector<unsigned char> z = {0x78, 0x01};
-    for (size_t off = 0; off < raw.size(); off += 65535) {
-        const size_t len = raw.size() - off < 65535 ? raw.size() - off : 65535;
-        z.push_back(off + len >= raw.size() ? 1 : 0);  // BFINAL, BTYPE = 00
-        z.push_back((unsigned char)(len & 0xff));
-        z.push_back((unsigned char)(len >> 8));
-        z.push_back((unsigned char)(~len & 0xff));
-        z.push_back((unsigned char)((~len >> 8) & 0xff));
-        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+    std::vector<unsigned char> z;
+    if (!png_zlib_compress(raw, z)) {
+        // zlib stream: header 0x78 0x01 (deflate, 32 KiB window), stored
+        // blocks of at most 65535 bytes, Adler-32 of the raw data
+        z = {0x78, 0x01};
+        for (size_t off = 0; off < raw.size(); off += 65535) {
+            const size_t len = raw.size() - off < 65535 ? raw.size() - off : 65535;
+            z.push_back(off + len >= raw.size() ? 1 : 0);  // BFINAL, BTYPE = 00
+            z.push_back((unsigned char)(len & 0xff));
+            z.push_back((unsigned char)(len >> 8));
+            z.push_back((unsigned char)(~len & 0xff));
+            z.push_back((unsigned char)((~len >> 8) & 0xff));
+            z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        }
+        uint32_t a = 1, b = 0;
+        for (unsigned char c : raw) {
+            a = (a + c) % 65521u;
+            b = (b + a) % 65521u;
+        }
+        png_put32(z, (b << 16) | a);
     }
-    uint32_t a = 1, b = 0;
-    for (unsigned char c : raw) {
-        a = (a + c) % 65521u;
-        b = (b + a) % 65521u;
-    }
-    png_put32(z, (b << 16) | a);
 
     std::vector<unsigned char> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
     std::vector<unsigned char> ihdr;

@@ -58,18 +58,26 @@ def test_viridis_table():
 
 
 @pytest.mark.parametrize("w,h", [(11, 7), (300, 100), (1, 1)])
-def test_cpp_frame_writer(tmp_path, w, h):
-    """include/fvm2d_frame.hpp (the drop-in header's frame writer, stored
-    deflate blocks: 300 x 100 needs two) decodes with zlib to the viridis
-    colouring of its input."""
+@pytest.mark.parametrize("stored", [False, True])
+def test_cpp_frame_writer(tmp_path, w, h, stored):
+    """include/fvm2d_frame.hpp (the drop-in header's frame writer: zlib's
+    compress2 loaded at run time, or stored deflate blocks, of which 300 x
+    100 needs two) decodes with zlib, and with PIL, to the viridis colouring
+    of its input."""
     exe = str(tmp_path / "frame_writer")
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "frame_writer.cpp"), "-o", exe], check=True)
     out = str(tmp_path / "f.png")
-    subprocess.run([exe, out, str(w), str(h)], check=True)
+    env = dict(os.environ, FVM2D_PNG_STORED="1" if stored else "0")
+    subprocess.run([exe, out, str(w), str(h)], check=True, env=env)
     i, j = np.meshgrid(np.arange(w), np.arange(h))
     gray = ((7 * i + 13 * j) % 256).astype(np.uint8)
     assert np.array_equal(io.read_png(out), io.colour_viridis(gray))
+    raw_bytes = h * (3 * w + 1)
+    if stored:
+        assert os.path.getsize(out) > raw_bytes
+    elif w * h > 1000:
+        assert os.path.getsize(out) < raw_bytes // 2  # compressed (libz loads on this host)
     try:
         from PIL import Image
     except ImportError:
