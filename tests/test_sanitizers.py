@@ -1,8 +1,9 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the
 oracle (every entry point, every preset, sizes that are not multiples of
-16, the host reference adjoint) and the engine's launch-layout logic and
-parameter checks, each built with g++ -fsanitize=address,undefined from a
-driver in tests/cpp/ and run; any report aborts the driver."""
+16, the host reference adjoint), the engine's launch-layout logic and
+parameter checks, and the drop-in header's PNG frame writer, each built
+with g++ -fsanitize=address,undefined from a driver in tests/cpp/ and run;
+any report aborts the driver."""
 import os
 import subprocess
 
@@ -36,3 +37,21 @@ def test_engine_layout_under_sanitizers(tmp_path):
     srcs = [os.path.join(ROOT, "tests", "cpp", "layout_sanitize.cpp")]
     _build_and_run(tmp_path, "layout_san", srcs, ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                                                   "-Wno-unused-result"])
+
+
+@pytest.mark.parametrize("stored", ["0", "1"])
+def test_frame_writer_under_sanitizers(tmp_path, stored):
+    """The drop-in header's PNG frame writer (include/fvm2d_frame.hpp), with
+    zlib loaded at run time and with stored blocks, on an image that needs
+    several stored blocks."""
+    exe = str(tmp_path / "frame_san")
+    r = subprocess.run(["g++", "-std=c++17", *SAN, "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "frame_writer.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, ASAN_OPTIONS="abort_on_error=1:detect_leaks=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1", FVM2D_PNG_STORED=stored)
+    out = str(tmp_path / "f.png")
+    r = subprocess.run([exe, out, "513", "301"], capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert os.path.getsize(out) > 0
