@@ -63,9 +63,12 @@ def write_frame_png(dir: str, rho: np.ndarray, index: int) -> str:
     rgb = colour_viridis(normalize_u8(rho))
     h, w = rgb.shape[:2]
     raw = b"".join(b"\x00" + rgb[j].tobytes() for j in range(h))  # filter type 0
+    z = zlib.compress(raw, 1)
+    idat = 1 << 30  # a chunk holds at most 2^31 - 1 bytes: 1 GiB IDAT chunks
     png = (b"\x89PNG\r\n\x1a\n" +
            _png_chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
-           _png_chunk(b"IDAT", zlib.compress(raw, 1)) + _png_chunk(b"IEND", b""))
+           b"".join(_png_chunk(b"IDAT", z[k:k + idat]) for k in range(0, len(z), idat)) +
+           _png_chunk(b"IEND", b""))
     path = os.path.join(dir, f"{index}.png")
     with open(path, "wb") as f:
         f.write(png)

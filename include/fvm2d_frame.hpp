@@ -121,7 +121,13 @@ inline void write_png_rgb(const std::string &path, int w, int h, const unsigned 
     png_put32(ihdr, (uint32_t)h);
     ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit, RGB, deflate, filter 0, no interlace
     png_chunk(out, "IHDR", ihdr);
-    png_chunk(out, "IDAT", z);
+    // a chunk holds at most 2^31 - 1 bytes: the zlib stream split over
+    // consecutive IDAT chunks of 1 GiB
+    const size_t kIdat = size_t(1) << 30;
+    for (size_t off = 0; off < z.size(); off += kIdat) {
+        const size_t n = z.size() - off < kIdat ? z.size() - off : kIdat;
+        png_chunk(out, "IDAT", std::vector<unsigned char>(z.begin() + off, z.begin() + off + n));
+    }
     png_chunk(out, "IEND", {});
     std::ofstream f(path, std::ios::binary);
     f.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)out.size());
