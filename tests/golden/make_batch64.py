@@ -1,11 +1,11 @@
-"""Writes config3_batch64.json (+ config3_batch64_sub.npz): the oracle's
+"""Writes batch64_128_1024.json (+ batch64_128_1024_sub.npz): the oracle's
 solve of each of the 64 batched-workload meshes (problems.batched_instances
 with seed 2024: sides ~U{128..1024}, non-square, seeded four-quadrant data,
 gamma 1.4, CFL 0.5, alpha 1.9, outflow), every mesh with its own dt for
 exactly 200 steps.  Per mesh: shape, step count, final time, SHA-256 of the
 final (rho, p, u, v) planes in [ny][nx][4] float64 order, and every 32nd
 cell of the final state.  About 4.4e9 cell-updates: ~15 minutes on 8 cores.
-    python tests/golden/make_config3.py [CACHE_DIR]
+    python tests/golden/make_batch64.py [CACHE_DIR]
 With CACHE_DIR, each mesh's result is kept there as it is computed and a
 rerun continues from the meshes already done.
 """
@@ -50,6 +50,6 @@ for i, U0 in enumerate(problems.batched_instances(count=COUNT, seed=SEED)):
 out = {"workload": f"{COUNT} meshes of problems.batched_instances(seed={SEED}), {STEPS} steps each, "
                    "CFL 0.5, alpha 1.9, gamma 1.4, outflow",
        "count": COUNT, "seed": SEED, "steps": STEPS, "stride": STRIDE, "meshes": recs}
-np.savez_compressed(os.path.join(HERE, "config3_batch64_sub.npz"), **subs)
-json.dump(out, open(os.path.join(HERE, "config3_batch64.json"), "w"), indent=1)
+np.savez_compressed(os.path.join(HERE, "batch64_128_1024_sub.npz"), **subs)
+json.dump(out, open(os.path.join(HERE, "batch64_128_1024.json"), "w"), indent=1)
 print("wall", time.time() - t0)

@@ -1,9 +1,9 @@
-"""Writes config2_2000.json (+ config2_2000_sub16.npy): the oracle's
+"""Writes fourquad_2000.json (+ fourquad_2000_sub16.npy): the oracle's
 2000x2000 four-quadrant solve to t_end = 0.3 (alpha 1.9, CFL 0.5, gamma 1.4,
 outflow): step count, final time, SHA-256 of the final (rho, p, u, v) planes
 in [ny][nx][4] float64 order, per-variable sums, and every 16th cell of the
 final state.  About 1.2e10 cell-updates: ~20 minutes on 8 cores.
-    python tests/golden/make_config2.py
+    python tests/golden/make_fourquad_2000.py
 """
 import hashlib
 import importlib
@@ -34,6 +34,6 @@ out = {
     "oracle_ms": ms,
     "oracle_threads": os.cpu_count(),
 }
-np.save(os.path.join(HERE, "config2_2000_sub16.npy"), np.ascontiguousarray(P4[::16, ::16]))
-json.dump(out, open(os.path.join(HERE, "config2_2000.json"), "w"), indent=1)
+np.save(os.path.join(HERE, "fourquad_2000_sub16.npy"), np.ascontiguousarray(P4[::16, ::16]))
+json.dump(out, open(os.path.join(HERE, "fourquad_2000.json"), "w"), indent=1)
 print(json.dumps(out, indent=1), "wall", time.time() - t0)

@@ -1,8 +1,8 @@
-"""Writes config1_256.json: the oracle's 256x256 four-quadrant solve to
+"""Writes fourquad_256.json: the oracle's 256x256 four-quadrant solve to
 t_end = 0.3 (alpha 1.9, CFL 0.5, gamma 1.4, outflow) as a regression digest
 (step count, SHA-256 of the final (rho, p, u, v) planes in [ny][nx][4]
 float64 order, and per-variable sums).  Run from the repository root:
-    python tests/golden/make_config1.py
+    python tests/golden/make_fourquad_256.py
 """
 import hashlib
 import importlib
@@ -28,6 +28,6 @@ out = {
     "sha256_prim4": hashlib.sha256(P4.tobytes()).hexdigest(),
     "sums_prim4": [float(x) for x in P4.reshape(-1, 4).sum(axis=0)],
 }
-path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config1_256.json")
+path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fourquad_256.json")
 json.dump(out, open(path, "w"), indent=1)
 print(json.dumps(out, indent=1))

@@ -3,12 +3,12 @@
 Batched forward: the 64 meshes of problems.batched_instances(seed 2024),
 sides 128..1024 (non-square, most not multiples of 16 or 64), 200 steps each
 with its own dt, all in one launch per step.  Every mesh is compared with
-the oracle's committed record (tests/golden/make_config3.py): EXACT mode bit
+the oracle's committed record (tests/golden/make_batch64.py): EXACT mode bit
 for bit (step count, SHA-256 of the final (rho, p, u, v) planes, every 32nd
 cell), FAST mode within relative L1 1e-10 of the EXACT result.
 
 8192 x 8192 plain forward: 3 steps of the four-quadrant data in EXACT mode
-against the oracle's record (tests/golden/make_config5.py), FAST within
+against the oracle's record (tests/golden/make_fourquad_8192.py), FAST within
 relative L1 1e-10.
 
 Batched adjoint: 32 meshes with sides 256..768 (the adjoint recipe under
@@ -35,9 +35,9 @@ GDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TOL_L1 = 1e-10
 
 
-def test_config3_batched_forward_vs_oracle(gpu, problems):
-    golden = json.load(open(os.path.join(GDIR, "config3_batch64.json")))
-    subs = np.load(os.path.join(GDIR, "config3_batch64_sub.npz"))
+def test_batch64_forward_vs_oracle(gpu, problems):
+    golden = json.load(open(os.path.join(GDIR, "batch64_128_1024.json")))
+    subs = np.load(os.path.join(GDIR, "batch64_128_1024_sub.npz"))
     stride, steps = golden["stride"], golden["steps"]
     Us = problems.batched_instances(count=golden["count"], seed=golden["seed"])
     shapes = [U.shape[:2] for U in Us]
@@ -72,9 +72,9 @@ def test_config3_batched_forward_vs_oracle(gpu, problems):
     print(f"FAST vs oracle, worst relative L1 over 64 meshes: {worst:.2e}")
 
 
-def test_config5_8192_forward_vs_oracle(gpu, problems):
-    golden = json.load(open(os.path.join(GDIR, "config5_8192.json")))
-    sub = np.load(os.path.join(GDIR, "config5_8192_sub64.npy"))
+def test_fourquad_8192_forward_vs_oracle(gpu, problems):
+    golden = json.load(open(os.path.join(GDIR, "fourquad_8192.json")))
+    sub = np.load(os.path.join(GDIR, "fourquad_8192_sub64.npy"))
     n, stride, steps = 8192, golden["stride"], golden["steps"]
     U0 = problems.initialize_4shocks(n)
     print("8192^2 EXACT", flush=True)
@@ -126,7 +126,7 @@ def _smooth_direction(rng, C):
     return v * np.abs(C)
 
 
-def test_config5_batched_adjoint_vs_single_and_fd(gpu, problems):
+def test_batch32_adjoint_vs_single_and_fd(gpu, problems):
     import importlib
     adjmod = importlib.import_module("2d_fvm_cuda_amd.adjoint")
     rng_s = problems.SplitMix64(77)
@@ -181,7 +181,7 @@ def test_config5_batched_adjoint_vs_single_and_fd(gpu, problems):
     assert max(abs(f - a) for f, a in zip(fds, ans)) <= 2e-5 * scale
 
 
-def test_config5_batched_adjoint_riemann_recipe_matches_single(gpu, problems):
+def test_batch32_adjoint_riemann_recipe_matches_single(gpu, problems):
     """32 meshes of seeded four-quadrant Riemann data (the batched forward's
     recipe, sides 256..768), 300 steps, 24 checkpoint slots: shocks from
     the first step, supersonic interfaces on both sides.  Forward states and

@@ -92,7 +92,7 @@ def test_fast_mode_parity(gpu, orc, cases, idx):
     assert np.all(l1 <= TOL_L1), (name, l1, linf)
 
 
-def test_config1_256(gpu, orc, problems):
+def test_fourquad_256(gpu, orc, problems):
     """256^2 four-quadrant to t_end = 0.3 (the reference's host-runnable case)."""
     U0 = problems.initialize_4shocks(256)
     prm = dict(t_end=0.3)
@@ -107,14 +107,14 @@ def test_config1_256(gpu, orc, problems):
     assert np.all(l1 <= TOL_L1)
 
 
-def test_config2_2000(gpu, problems):
+def test_fourquad_2000(gpu, problems):
     """2000^2 four-quadrant to t_end = 0.3, the headline workload, against
-    the oracle's committed record (tests/golden/make_config2.py): EXACT mode
+    the oracle's committed record (tests/golden/make_fourquad_2000.py): EXACT mode
     bit for bit (step count, SHA-256 of the final (rho, p, u, v) planes and
     every 16th cell), FAST mode within relative L1 1e-10 of it."""
     gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    golden = json.load(open(os.path.join(gdir, "config2_2000.json")))
-    sub16 = np.load(os.path.join(gdir, "config2_2000_sub16.npy"))
+    golden = json.load(open(os.path.join(gdir, "fourquad_2000.json")))
+    sub16 = np.load(os.path.join(gdir, "fourquad_2000_sub16.npy"))
     U0 = problems.initialize_4shocks(2000)
     ex, steps_ex, _ = gpu.solve(U0, mode=gpu.MODE_EXACT, t_end=0.3)
     assert steps_ex == golden["steps"] == 3716

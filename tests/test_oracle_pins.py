@@ -413,7 +413,7 @@ def test_interior_mass_conserved_before_waves_reach_ghosts(orc, problems):
     assert abs(m1 - m0) / m0 < 1e-13
 
 
-def test_step_count_config1(orc, problems):
+def test_step_count_fourquad_256(orc, problems):
     """256^2 four-quadrant to t_end = 0.3 takes 403 steps and stays
     positive; the final-state digest is the committed regression fixture."""
     import hashlib
@@ -423,7 +423,7 @@ def test_step_count_config1(orc, problems):
     U, n, t, _ = orc.solve(U0, nthreads=0, t_end=0.3)
     assert n == 403 and t == pytest.approx(0.3, abs=0)
     assert np.all(U[..., 0] > 0) and np.all(U[..., 1] > 0)
-    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1_256.json")))
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fourquad_256.json")))
     assert fx["steps"] == n
     assert hashlib.sha256(np.ascontiguousarray(U[..., :4]).tobytes()).hexdigest() == fx["sha256_prim4"]
 
