@@ -20,10 +20,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <deque>
 #include <filesystem>
 #include <fstream>
+#include <future>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fvm2d.h"
@@ -149,12 +152,18 @@ class RGRP_2D_CUDA {
     // step and every gap_ further steps the state is copied out as the
     // reference copies it (after the branch average, before the ghost fill:
     // fvm2d_download_frame) and its density written as
-    // <recordDir_>/<step index>.png.
+    // <recordDir_>/<step index>.png.  The 8-bit image is made here; its
+    // colouring, compression and file write run on worker threads (at most
+    // one per hardware thread, up to 16, in flight) while the device steps
+    // on.  Like the reference's, the computation time includes writing the
+    // frames: it ends when the last one is written.
     void solve_recording(const fvm2d_params &p, int *steps_out, double *ms_out) {
         fvm2d_solver *sv = nullptr;
         check(fvm2d_create(Nx_, Ny_, &p, &sv), "fvm2d_create");
         std::vector<vec5d> frame((size_t)Nx_ * Ny_);
         int steps = 0;
+        const size_t max_pending = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::deque<std::future<void>> pending;
         try {
             check(fvm2d_upload(sv, reinterpret_cast<const double *>(U0_.data())), "fvm2d_upload");
             std::filesystem::create_directories(recordDir_);
@@ -170,12 +179,25 @@ class RGRP_2D_CUDA {
                 done = steps;
                 check(fvm2d_download_frame(sv, reinterpret_cast<double *>(frame.data())),
                       "fvm2d_download_frame");
-                write_frame(frame, due - 1);
+                if (pending.size() >= max_pending) {
+                    pending.front().get();
+                    pending.pop_front();
+                }
+                pending.push_back(std::async(std::launch::async,
+                                             [path = recordDir_ + "/" + std::to_string(due - 1) + ".png",
+                                              gray = frame_gray(frame), w = Nx_, h = Ny_] {
+                                                 fvm2d::write_frame_png(path, w, h, gray.data());
+                                             }));
                 due += gap;
+            }
+            while (!pending.empty()) {
+                pending.front().get();
+                pending.pop_front();
             }
             *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             check(fvm2d_download(sv, reinterpret_cast<double *>(U0_.data())), "fvm2d_download");
         } catch (...) {
+            pending.clear();  // each future's destructor waits for its write
             fvm2d_destroy(sv);
             throw;
         }
@@ -184,22 +206,22 @@ class RGRP_2D_CUDA {
     }
 
     // Density min-max normalised to 0..255 and rounded (the
-    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340),
-    // coloured with viridis and written as <recordDir_>/<index>.png
+    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340);
+    // fvm2d::write_frame_png colours it with viridis and writes the PNG
     // (src/wrappers.cuh:342-344).
-    void write_frame(const std::vector<vec5d> &f, int index) const {
+    std::vector<unsigned char> frame_gray(const std::vector<vec5d> &f) const {
         double lo = f[0](0), hi = f[0](0);
         for (const auto &e : f) {
             lo = std::min(lo, e(0));
             hi = std::max(hi, e(0));
         }
         const double scale = hi > lo ? 255.0 / (hi - lo) : 0.0;
-        std::vector<unsigned char> gray((size_t)Nx_ * Ny_);
+        std::vector<unsigned char> gray(f.size());
         for (size_t k = 0; k < gray.size(); ++k) {
             const double v = std::nearbyint(f[k](0) * scale - lo * scale);
             gray[k] = (unsigned char)std::min(255.0, std::max(0.0, v));
         }
-        fvm2d::write_frame_png(recordDir_ + "/" + std::to_string(index) + ".png", Nx_, Ny_, gray.data());
+        return gray;
     }
 
     std::vector<std::vector<vec5d>> vectorized() const {

@@ -1,6 +1,7 @@
 // test2D-style caller (test2D.cu:46-80, 135-152) of the drop-in header.
-// Usage: drop_in_caller N t_end [mode [record_dir|- gap [txt_dir]]]
-//   -> prints "steps ms rho_sum"; with txt_dir, writeFinalResultTo_txt(txt_dir)
+// Usage: drop_in_caller N t_end [mode [record_dir|- gap [txt_dir|- [alpha]]]]
+//   -> prints "steps ms rho_sum"; with txt_dir, writeFinalResultTo_txt(txt_dir);
+//   alpha defaults to 1.9 (test2D.cu's main uses 2.0 and record_dir ./pngs, gap 18)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -30,7 +31,7 @@ int main(int argc, char **argv) {
     solver.setSpatialLayout(1.0, 1.0);
     solver.setTimeLayout(tend, 0.5);
     solver.setGhostCellStrategy(ghostCellStrategy::outflow);
-    solver.setAlpha(1.9);
+    solver.setAlpha(argc > 7 ? std::atof(argv[7]) : 1.9);
     solver.setGamma(1.4);
     if (argc > 3) solver.setArithmeticMode(std::atoi(argv[3]));
     if (argc > 5 && std::strcmp(argv[4], "-") != 0) solver.setRecordModel(true, std::atoi(argv[5]), argv[4]);
@@ -39,7 +40,7 @@ int main(int argc, char **argv) {
     for (auto &row : U)
         for (auto &c : row) s += c(0);
     std::printf("%d %.3f %.17g\n", solver.getTimeStagesCount(), solver.getComputationTime_ms(), s);
-    if (argc > 6) solver.writeFinalResultTo_txt(argv[6]);
+    if (argc > 6 && std::strcmp(argv[6], "-") != 0) solver.writeFinalResultTo_txt(argv[6]);
     solver.clear();
     return 0;
 }
