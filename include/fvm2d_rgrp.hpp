@@ -62,11 +62,17 @@ enum class ghostCellStrategy { outflow, reflective, jet, jet800, doubleMach };
 
 class RGRP_2D_CUDA {
   public:
+    // Rows of unequal length (undefined in the reference) are refused with
+    // std::invalid_argument; an empty U0 gives a 0 x 0 mesh, which solve()
+    // refuses.
     explicit RGRP_2D_CUDA(const std::vector<std::vector<vec5d>> &U0) {
-        for (auto &row : U0)
-            for (auto &u : row) U0_.push_back(u);
-        Nx_ = (int)U0[0].size();
         Ny_ = (int)U0.size();
+        Nx_ = U0.empty() ? 0 : (int)U0[0].size();
+        for (auto &row : U0) {
+            if (row.size() != (size_t)Nx_)
+                throw std::invalid_argument("RGRP_2D_CUDA: the rows of U0 differ in length");
+            for (auto &u : row) U0_.push_back(u);
+        }
     }
     RGRP_2D_CUDA() {}
 

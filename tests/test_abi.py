@@ -94,3 +94,19 @@ def test_drop_in_header_compiles_against_abi():
                         "-L", lib_dir, "-lfvm2d", "-Wl,-rpath," + lib_dir, "-o", out],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_drop_in_header_argument_errors(tmp_path):
+    """The C++ drop-in header refuses U0 rows of unequal length
+    (std::invalid_argument) and an empty mesh (solve() throws
+    std::runtime_error from fvm2d_solve's argument check, before any HIP
+    call)."""
+    src = os.path.join(ROOT, "tests", "cpp", "header_errors.cpp")
+    out = str(tmp_path / "header_errors")
+    lib_dir = os.path.join(ROOT, "2d_fvm_cuda_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src,
+                        "-L", lib_dir, "-lfvm2d", "-Wl,-rpath," + lib_dir, "-o", out],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
