@@ -72,6 +72,15 @@ def test_bad_arguments_are_rejected_without_device(fvm):
     rc = fvm.lib().fvm2d_create(4, 100, ctypes.byref(p), ctypes.byref(h))
     assert rc == -1
     assert b"6x6" in fvm.lib().fvm2d_last_error()
+    # cell indices are 32-bit signed, as the reference's idx = j * Nx + i
+    rc = fvm.lib().fvm2d_create(46341, 46341, ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and b"2^31-1" in fvm.lib().fvm2d_last_error()
+    for field, bad, msg in (("strategy", 5, b"strategy"), ("mode", 2, b"mode"), ("cfl", 0.0, b"CFL"),
+                            ("width", -1.0, b"width"), ("alpha", float("nan"), b"NaN")):
+        q = fvm.make_params(t_end=0.1)
+        setattr(q, field, bad)
+        rc = fvm.lib().fvm2d_create(64, 64, ctypes.byref(q), ctypes.byref(h))
+        assert rc == -1 and msg in fvm.lib().fvm2d_last_error(), field
     p.gamma = 1.0
     rc = fvm.lib().fvm2d_create(64, 64, ctypes.byref(p), ctypes.byref(h))
     assert rc == -1 and b"gamma" in fvm.lib().fvm2d_last_error()
