@@ -51,15 +51,19 @@ def test_host_forward_matches_oracle(orc, problems, strategy):
     np.testing.assert_allclose(got, to_cons(ref), rtol=1e-11, atol=1e-12)
 
 
-@pytest.mark.parametrize("strategy", [0, 3, 4])
-def test_host_gradient_directional_fd(orc, problems, strategy):
+@pytest.mark.parametrize("strategy,gas", [(0, {}), (3, {}), (4, {}),
+                                          (0, dict(gamma=5.0 / 3.0, alpha=2.0, cfl=0.4)),
+                                          (4, dict(gamma=1.1, alpha=1.5))])
+def test_host_gradient_directional_fd(orc, problems, strategy, gas):
     """<dL/dU0, v> = (L(U0 + eps v) - L(U0 - eps v)) / (2 eps) for
-    L = <w, U_N> with random w and a random smooth direction v."""
+    L = <w, U_N> with random w and a random smooth direction v; also at
+    gamma 5/3 (alpha 2, the demo's value; CFL 0.4) and gamma 1.1."""
     C0 = small_problem(problems, 22, 18, seed=11 + strategy)
     n = 6
     rng = np.random.default_rng(strategy)
     w = rng.normal(size=C0.shape)
-    g0 = orc.adj_backward(C0, n, w, strategy=strategy)
+    kw = dict(strategy=strategy, **gas)
+    g0 = orc.adj_backward(C0, n, w, **kw)
     for trial in range(3):
         v = rng.normal(size=C0.shape) * np.abs(C0) * 1e-2
         ad = np.sum(g0 * v)
@@ -68,8 +72,7 @@ def test_host_gradient_directional_fd(orc, problems, strategy):
         # a step may straddle a limiter or GRP branch switch of one cell, so
         # the better of two step sizes is taken
         for eps in (1e-5, 1e-6):
-            d = orc.adj_forward(C0 + eps * v, n, strategy=strategy) - orc.adj_forward(
-                C0 - eps * v, n, strategy=strategy)
+            d = orc.adj_forward(C0 + eps * v, n, **kw) - orc.adj_forward(C0 - eps * v, n, **kw)
             fd = np.sum(w * d) / (2 * eps)  # difference per cell first
             errs.append(abs(fd - ad))
         assert min(errs) <= 1e-6 * scale, (trial, errs, scale)

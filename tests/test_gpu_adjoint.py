@@ -32,17 +32,22 @@ def smooth_state(problems, nx, ny, seed, vel=True):
     return to_cons(U5)
 
 
-@pytest.mark.parametrize("shape,strategy,steps", [((40, 36), 0, 10), ((61, 130), 0, 7),
-                                                  ((120, 70), 3, 6), ((50, 64), 4, 5),
-                                                  ((33, 47), 2, 4), ((44, 70), 1, 5)])
-def test_gpu_gradient_matches_host_reference(gpu, orc, problems, adjmod, shape, strategy, steps):
+@pytest.mark.parametrize("shape,strategy,steps,gas", [((40, 36), 0, 10, {}), ((61, 130), 0, 7, {}),
+                                                      ((120, 70), 3, 6, {}), ((50, 64), 4, 5, {}),
+                                                      ((33, 47), 2, 4, {}), ((44, 70), 1, 5, {}),
+                                                      ((48, 56), 0, 8, dict(gamma=5.0 / 3.0, alpha=2.0,
+                                                                            cfl=0.4)),
+                                                      ((52, 45), 4, 6, dict(gamma=1.1, alpha=1.5))])
+def test_gpu_gradient_matches_host_reference(gpu, orc, problems, adjmod, shape, strategy, steps, gas):
+    """Also at gamma 5/3 (alpha 2.0, the demo's value; CFL 0.4) and gamma 1.1:
+    the FAST closed forms fold gamma into constants."""
     ny, nx = shape
-    print(f"case shape {shape} strategy {strategy} steps {steps}", flush=True)
+    print(f"case shape {shape} strategy {strategy} steps {steps} {gas}", flush=True)
     C0 = smooth_state(problems, nx, ny, seed=nx + ny)
     rng = np.random.default_rng(nx)
     gN = rng.normal(size=C0.shape)
     # (REFLECTIVE runs the double-Mach preset, as the reference's solve does)
-    prm = dict(strategy=strategy, width=nx / ny if strategy in (1, 2, 3, 4) else 1.0)
+    prm = dict(strategy=strategy, width=nx / ny if strategy in (1, 2, 3, 4) else 1.0, **gas)
     ref_UN = orc.adj_forward(C0, steps, **prm)
     ref_g0 = orc.adj_backward(C0, steps, gN, **prm)
     adj = adjmod.Adjoint([shape], [prm], nsteps=steps, checkpoint_slots=3)
