@@ -84,8 +84,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the y-strip multi-GPU decomposition is a design, not code", file=sys.stderr)
+        print(f"bench.py: fvm2d solves on one GPU only (asked for --gpus {args.gpus}); "
+              "a row-strip decomposition over several GPUs is designed but not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     fvm = importlib.import_module("2d_fvm_cuda_amd")
