@@ -267,6 +267,34 @@ def test_batch_matches_oracle(gpu, orc, problems):
         b.close()
 
 
+def test_batch_of_mixed_cases_matches_oracle(gpu, orc, cases):
+    """Every parity case in one batch: per-instance presets (outflow,
+    reflective, jet, jet800, double Mach), gamma, alpha, CFL, domain size,
+    t_end and step limit, meshes from 6x6 to 6x3001 and 2999x7 side by side
+    in each launch.  Each instance equals its own oracle solve: bitwise in
+    EXACT mode, within relative L1 1e-10 in FAST mode."""
+    refs = [orc.solve(U0, nthreads=4, **prm) for _, U0, prm in cases]
+    for mode in (1, 0):
+        b = gpu.Batch([U0.shape[:2] for _, U0, _ in cases], [dict(prm, mode=mode) for _, _, prm in cases])
+        for i, (_, U0, _) in enumerate(cases):
+            b.upload(i, U0)
+        b.begin()
+        b.run(100000)
+        for i, (name, _, prm) in enumerate(cases):
+            ref, n_ref, _, _ = refs[i]
+            n, _ = b.status(i)
+            got = b.download(i)
+            print(f"mode {mode} case {name}: steps {n}/{n_ref}", flush=True)
+            if mode == 1:
+                assert n == n_ref, name
+                assert np.array_equal(got[..., :4], ref[..., :4]), name
+            else:
+                assert abs(n - n_ref) <= 1, name
+                l1, linf = rel_errors(got, ref, prm.get("gamma", 1.4))
+                assert np.all(l1 <= TOL_L1), (name, l1, linf)
+        b.close()
+
+
 def test_autotune_keeps_state_and_results(gpu, problems):
     """fvm2d_autotune / fvm2d_batch_autotune time candidate launch shapes on
     scratch copies: the resident state is unchanged afterwards, the chosen
