@@ -65,6 +65,7 @@ EXPORTS = [
     "fvm2d_solve_device_noalloc",
     "fvm2d_create", "fvm2d_set_params", "fvm2d_upload", "fvm2d_upload_planes_device",
     "fvm2d_download_planes_device", "fvm2d_run", "fvm2d_download", "fvm2d_download_frame",
+    "fvm2d_download_frame_gray",
     "fvm2d_steps_total",
     "fvm2d_destroy", "fvm2d_set_launch", "fvm2d_get_launch", "fvm2d_autotune", "fvm2d_begin",
     "fvm2d_advance", "fvm2d_sync", "fvm2d_time_steps", "fvm2d_time_kernel",
@@ -121,6 +122,7 @@ def lib():
         "fvm2d_run": (ctypes.c_int, [_vp, _ip, _dp, _dp]),
         "fvm2d_download": (ctypes.c_int, [_vp, _dp]),
         "fvm2d_download_frame": (ctypes.c_int, [_vp, _dp]),
+        "fvm2d_download_frame_gray": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_ubyte)]),
         "fvm2d_steps_total": (ctypes.c_int, [_vp]),
         "fvm2d_destroy": (ctypes.c_int, [_vp]),
         "fvm2d_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -313,6 +315,14 @@ class Solver:
         U = np.empty((self.ny, self.nx, 5))
         check(lib().fvm2d_download_frame(self.h, _ptr(U)), "fvm2d_download_frame")
         return U
+
+    def download_frame_gray(self) -> np.ndarray:
+        """The density of download_frame() as the reference's 8-bit frame
+        image (min-max normalised and rounded on the device): (ny, nx) uint8."""
+        g = np.empty((self.ny, self.nx), dtype=np.uint8)
+        check(lib().fvm2d_download_frame_gray(self.h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))),
+              "fvm2d_download_frame_gray")
+        return g
 
     def upload_planes_device(self, ptr: int):
         check(lib().fvm2d_upload_planes_device(self.h, _vp(ptr)), "fvm2d_upload_planes_device")
@@ -508,13 +518,15 @@ class RGRP_2D_CUDA:
         named by the index of the step (0, gap, 2 gap, ...).  The loop runs
         on the device in chunks of gapPerFrame steps; each frame is the
         state as the reference copies it out, after the branch average and
-        before that step's ghost fill (fvm2d_download_frame).  Frames are
-        encoded and written by a pool of host threads while the device
-        steps on; like the reference's, the computation time includes
-        writing them (the call returns after the last one is written)."""
+        before that step's ghost fill; its density is normalised to the
+        8-bit image on the device (fvm2d_download_frame_gray).  Frames are
+        coloured, encoded and written by a pool of host threads while the
+        device steps on; like the reference's, the computation time
+        includes writing them (the call returns after the last one is
+        written)."""
         import time
         from concurrent.futures import ThreadPoolExecutor
-        from .io import write_frame_png
+        from .io import write_gray_png
         gap = max(1, self._gap)
         workers = max(1, min(16, os.cpu_count() or 1))
         s = Solver(self._Nx, self._Ny, width=self._width, height=self._height,
@@ -533,8 +545,8 @@ class RGRP_2D_CUDA:
                     if steps < due:
                         break
                     done_steps = steps
-                    rho = np.ascontiguousarray(s.download_frame()[..., 0])
-                    pending.append(pool.submit(write_frame_png, self._recordDir, rho, due - 1))
+                    gray = s.download_frame_gray()
+                    pending.append(pool.submit(write_gray_png, self._recordDir, gray, due - 1))
                     if len(pending) >= 2 * workers:  # bound the frames held in memory
                         pending.pop(0).result()
                     due += gap

@@ -87,6 +87,8 @@ struct Engine {
     int gsteps = 0;                  // steps captured in gexec
     double *stage = nullptr;         // pinned staging for one instance
     double *d_frame = nullptr;       // record-mode frame (4 planes of the largest mesh)
+    unsigned char *d_gray = nullptr; // its 8-bit density image
+    unsigned long long *d_range = nullptr;  // the image's min/max keys
     long long stage_doubles = 0;
     std::vector<int> steps_total;
     bool begun = false;
@@ -111,6 +113,10 @@ struct Engine {
         if (stage) hipHostFree(stage);
         if (d_frame) hipFree(d_frame);
         d_frame = nullptr;
+        if (d_gray) hipFree(d_gray);
+        d_gray = nullptr;
+        if (d_range) hipFree(d_range);
+        d_range = nullptr;
         if (e0) hipEventDestroy(e0);
         if (e1) hipEventDestroy(e1);
         if (stream) hipStreamDestroy(stream);
@@ -464,6 +470,36 @@ struct Engine {
     // overwritten it), and dt_used is the step's own dt.
     int frame(int i, double *U5) {
         if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
+        const int rc = make_frame(i);
+        return rc ? rc : copy_out(i, d_frame, U5);
+    }
+
+    // The frame's density as the 8-bit image of the reference's frame file
+    // (min-max normalised and rounded on the device), gray[j * nx + i]:
+    // nx * ny bytes cross to the host instead of the whole state.
+    int frame_gray(int i, unsigned char *gray) {
+        if (i < 0 || i >= n || !gray) return fail(FVM2D_ERR_ARG, "bad instance or null image");
+        int rc = make_frame(i);
+        if (rc) return rc;
+        const InstDesc &d = hd[i];
+        if (!d_gray) {
+            long long mx = 0;
+            for (const auto &e : hd) mx = std::max(mx, (long long)e.nx * e.ny);
+            HIPCHK(hipMalloc(&d_gray, mx));
+            HIPCHK(hipMalloc(&d_range, 2 * sizeof(unsigned long long)));
+        }
+        launch_frame_gray(d, d_frame, d_range, d_gray, stream);
+        HIPCHK(hipGetLastError());
+        // through the pinned staging buffer (it holds a whole state)
+        const size_t bytes = (size_t)d.nx * d.ny;
+        HIPCHK(hipMemcpyAsync(stage, d_gray, bytes, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::memcpy(gray, stage, bytes);
+        return FVM2D_OK;
+    }
+
+    // The pre-fill frame of instance i into d_frame (see frame()).
+    int make_frame(int i) {
         if (!begun) return fail(FVM2D_ERR_STATE, "a frame needs a time loop (begin/advance)");
         int rc = read_ctl();
         if (rc) return rc;
@@ -479,7 +515,7 @@ struct Engine {
         const double *prev = (c.steps & 1) ? d.buf0 : d.buf1;
         launch_frame(exact, d, prev, cur, c.dt_used, d_frame, stream);
         HIPCHK(hipGetLastError());
-        return copy_out(i, d_frame, U5);
+        return FVM2D_OK;
     }
 
     int planes_device(int i, double *d_planes, bool in) {

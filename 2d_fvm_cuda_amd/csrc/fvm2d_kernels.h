@@ -19,6 +19,10 @@ void launch_ctl_restore(Ctl *ctls, const Ctl *saved, int ninst, hipStream_t stre
 // fill) from that step's input prev and output cur, into out (4 planes).
 void launch_frame(bool exact, const InstDesc &d, const double *prev, const double *cur, double dt,
                   double *out, hipStream_t stream);
+// The 8-bit image of a frame's density (min-max normalised, rounded):
+// gray[j * nx + i]; range holds two 64-bit words of scratch.
+void launch_frame_gray(const InstDesc &d, const double *frame, unsigned long long *range,
+                       unsigned char *gray, hipStream_t stream);
 int step_kernel_occupancy(bool exact, int wpb);
 
 }  // namespace fvm2d

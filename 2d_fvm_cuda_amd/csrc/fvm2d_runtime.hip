@@ -109,6 +109,11 @@ int fvm2d_download_frame(fvm2d_solver *s, double *U5) {
     return s->eng.frame(0, U5);
 }
 
+int fvm2d_download_frame_gray(fvm2d_solver *s, unsigned char *gray) {
+    if (!s) return fail(FVM2D_ERR_ARG, "solver is null");
+    return s->eng.frame_gray(0, gray);
+}
+
 int fvm2d_steps_total(const fvm2d_solver *s) { return s ? s->eng.steps_total[0] : 0; }
 
 int fvm2d_destroy(fvm2d_solver *s) {

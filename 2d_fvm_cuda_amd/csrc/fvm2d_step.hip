@@ -460,6 +460,75 @@ void launch_frame(bool exact, const InstDesc &d, const double *prev, const doubl
         hipLaunchKernelGGL(fvm_frame_kernel<false>, dim3(blocks), dim3(256), 0, stream, d, prev, cur, dt, out);
 }
 
+// ---------------------------------------------------------------- 8-bit frame image
+// The cv::normalize(..., 0, 255, NORM_MINMAX, CV_8UC1) step of the
+// reference's frame writer (src/wrappers.cuh:323-340) on a frame's density:
+// lo and hi over every cell (ghosts included), s = 255 / (hi - lo), pixel =
+// rint(rho s - lo s) clamped to 0..255; every pixel 0 when hi == lo.  NaN
+// densities are left out of lo and hi and give 0.
+//
+// Order-preserving 64-bit keys of doubles (non-NaN): min and max are exact,
+// so atomics in any order give the same lo and hi.
+__device__ __forceinline__ unsigned long long order_key(double x) {
+    const unsigned long long b = dbits(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
+__global__ void fvm_range_init_kernel(unsigned long long *range) {
+    if (threadIdx.x == 0) {
+        range[0] = ~0ull;  // min key
+        range[1] = 0ull;   // max key
+    }
+}
+
+__global__ void __launch_bounds__(256) fvm_range_kernel(InstDesc d, const double *__restrict__ frame,
+                                                        unsigned long long *range) {
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x)
+        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+            const double r = frame[(long long)j * d.pitch + i];
+            if (r != r) continue;
+            const unsigned long long key = order_key(r);
+            lo = key < lo ? key : lo;
+            hi = key > hi ? key : hi;
+        }
+    lo = ~wave_max_u64(~lo);
+    hi = wave_max_u64(hi);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicMin(&range[0], lo);
+        atomicMax(&range[1], hi);
+    }
+}
+
+__global__ void __launch_bounds__(256) fvm_gray_kernel(InstDesc d, const double *__restrict__ frame,
+                                                       const unsigned long long *__restrict__ range,
+                                                       unsigned char *__restrict__ gray) {
+    const unsigned long long klo = range[0], khi = range[1];
+    const bool any = klo <= khi;
+    const double lo = any ? key_value(klo) : 0.0, hi = any ? key_value(khi) : 0.0;
+    const double s = hi > lo ? 255.0 / (hi - lo) : 0.0;
+    const double ls = lo * s;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x)
+        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+            double v = rint(frame[(long long)j * d.pitch + i] * s - ls);
+            v = v > 0.0 ? v : 0.0;  // NaN -> 0
+            v = v < 255.0 ? v : 255.0;
+            gray[(long long)j * d.nx + i] = (unsigned char)v;
+        }
+}
+
+void launch_frame_gray(const InstDesc &d, const double *frame, unsigned long long *range,
+                       unsigned char *gray, hipStream_t stream) {
+    const unsigned blocks = (unsigned)std::min(d.ny, 4096);
+    hipLaunchKernelGGL(fvm_range_init_kernel, dim3(1), dim3(64), 0, stream, range);
+    hipLaunchKernelGGL(fvm_range_kernel, dim3(blocks), dim3(256), 0, stream, d, frame, range);
+    hipLaunchKernelGGL(fvm_gray_kernel, dim3(blocks), dim3(256), 0, stream, d, frame, range, gray);
+}
+
 // ---------------------------------------------------------------- launchers
 void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream) {

@@ -59,8 +59,15 @@ def write_frame_png(dir: str, rho: np.ndarray, index: int) -> str:
     (ghost cells included, row 0 first) after normalize_u8, coloured with
     viridis, as an 8-bit RGB PNG -- the file src/wrappers.cuh:323-345 writes
     through OpenCV (zlib level 1, OpenCV's default for PNG)."""
+    return write_gray_png(dir, normalize_u8(rho), index)
+
+
+def write_gray_png(dir: str, gray: np.ndarray, index: int) -> str:
+    """``<dir>/<index>.png`` from the frame's normalised 8-bit image (as
+    normalize_u8 or the device's fvm2d_download_frame_gray makes it),
+    coloured with viridis."""
     os.makedirs(dir, exist_ok=True)
-    rgb = colour_viridis(normalize_u8(rho))
+    rgb = colour_viridis(gray)
     h, w = rgb.shape[:2]
     raw = b"".join(b"\x00" + rgb[j].tobytes() for j in range(h))  # filter type 0
     z = zlib.compress(raw, 1)

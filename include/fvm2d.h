@@ -108,6 +108,17 @@ int fvm2d_download(fvm2d_solver *s, double *U5);
  * src/wrappers.cuh:265-281).  Layout as fvm2d_download.  FVM2D_ERR_STATE
  * outside a time loop or before its first step. */
 int fvm2d_download_frame(fvm2d_solver *s, double *U5);
+/* No counterpart in the reference (its frame writer does this on the host
+ * with OpenCV, src/wrappers.cuh:323-340): the density of the frame of
+ * fvm2d_download_frame as the 8-bit image the reference's frame file holds
+ * before its colour map, computed on the device so that nx * ny bytes cross
+ * instead of the state.  gray: ny x nx bytes, row 0 first.  lo and hi are
+ * the minimum and maximum density over every cell (ghosts included), s =
+ * 255 / (hi - lo), pixel = rint(rho s - lo s) clamped to 0..255 (round half
+ * to even, each operation rounded once); all pixels 0 when hi == lo; NaN
+ * densities are left out of lo and hi and give 0.  Errors as
+ * fvm2d_download_frame. */
+int fvm2d_download_frame_gray(fvm2d_solver *s, unsigned char *gray);
 /* Total time steps over all runs (getTimeStagesCount, src/wrappers.cuh:313-316). */
 int fvm2d_steps_total(const fvm2d_solver *s);
 int fvm2d_destroy(fvm2d_solver *s);

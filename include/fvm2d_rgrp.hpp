@@ -155,18 +155,18 @@ class RGRP_2D_CUDA {
     }
 
     // The time loop on the device in chunks of gap_ steps; after the first
-    // step and every gap_ further steps the state is copied out as the
-    // reference copies it (after the branch average, before the ghost fill:
-    // fvm2d_download_frame) and its density written as
-    // <recordDir_>/<step index>.png.  The 8-bit image is made here; its
-    // colouring, compression and file write run on worker threads (at most
-    // one per hardware thread, up to 16, in flight) while the device steps
-    // on.  Like the reference's, the computation time includes writing the
-    // frames: it ends when the last one is written.
+    // step and every gap_ further steps the density of the state as the
+    // reference copies it out (after the branch average, before the ghost
+    // fill) is written as <recordDir_>/<step index>.png.  Its 8-bit image
+    // (the min-max normalisation of src/wrappers.cuh:340) is made on the
+    // device (fvm2d_download_frame_gray); the colouring, compression and
+    // file write run on worker threads (at most one per hardware thread,
+    // up to 16, in flight) while the device steps on.  Like the
+    // reference's, the computation time includes writing the frames: it
+    // ends when the last one is written.
     void solve_recording(const fvm2d_params &p, int *steps_out, double *ms_out) {
         fvm2d_solver *sv = nullptr;
         check(fvm2d_create(Nx_, Ny_, &p, &sv), "fvm2d_create");
-        std::vector<vec5d> frame((size_t)Nx_ * Ny_);
         int steps = 0;
         const size_t max_pending = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::deque<std::future<void>> pending;
@@ -183,15 +183,15 @@ class RGRP_2D_CUDA {
                 check(fvm2d_sync(sv, &steps, &t, &dt), "fvm2d_sync");
                 if (steps < due) break;
                 done = steps;
-                check(fvm2d_download_frame(sv, reinterpret_cast<double *>(frame.data())),
-                      "fvm2d_download_frame");
+                std::vector<unsigned char> gray((size_t)Nx_ * Ny_);
+                check(fvm2d_download_frame_gray(sv, gray.data()), "fvm2d_download_frame_gray");
                 if (pending.size() >= max_pending) {
                     pending.front().get();
                     pending.pop_front();
                 }
                 pending.push_back(std::async(std::launch::async,
                                              [path = recordDir_ + "/" + std::to_string(due - 1) + ".png",
-                                              gray = frame_gray(frame), w = Nx_, h = Ny_] {
+                                              gray = std::move(gray), w = Nx_, h = Ny_] {
                                                  fvm2d::write_frame_png(path, w, h, gray.data());
                                              }));
                 due += gap;
@@ -209,25 +209,6 @@ class RGRP_2D_CUDA {
         }
         fvm2d_destroy(sv);
         *steps_out = steps;
-    }
-
-    // Density min-max normalised to 0..255 and rounded (the
-    // cv::normalize(..., NORM_MINMAX, CV_8UC1) step of src/wrappers.cuh:340);
-    // fvm2d::write_frame_png colours it with viridis and writes the PNG
-    // (src/wrappers.cuh:342-344).
-    std::vector<unsigned char> frame_gray(const std::vector<vec5d> &f) const {
-        double lo = f[0](0), hi = f[0](0);
-        for (const auto &e : f) {
-            lo = std::min(lo, e(0));
-            hi = std::max(hi, e(0));
-        }
-        const double scale = hi > lo ? 255.0 / (hi - lo) : 0.0;
-        std::vector<unsigned char> gray(f.size());
-        for (size_t k = 0; k < gray.size(); ++k) {
-            const double v = std::nearbyint(f[k](0) * scale - lo * scale);
-            gray[k] = (unsigned char)std::min(255.0, std::max(0.0, v));
-        }
-        return gray;
     }
 
     std::vector<std::vector<vec5d>> vectorized() const {

@@ -51,12 +51,32 @@ def test_frames_equal_oracle_prefill_states(gpu, orc, problems, strategy):
     s.begin()
     with pytest.raises(gpu.FVM2DError, match="no step"):
         s.download_frame()
+    with pytest.raises(gpu.FVM2DError, match="no step"):
+        s.download_frame_gray()
     for k in range(n):
         s.advance(1)
         steps, _, _ = s.sync()
         assert steps == k + 1
         got = s.download_frame()
         assert np.array_equal(got[..., :4], frames[k][..., :4]), k
+        # the device's 8-bit image of the same frame
+        assert np.array_equal(s.download_frame_gray(), io.normalize_u8(frames[k][..., 0])), k
+    s.close()
+
+
+def test_frame_gray_of_a_constant_density_is_zero(gpu, problems):
+    """hi == lo: every pixel 0 (as normalize_u8 and cv::normalize give)."""
+    U0 = np.zeros((20, 30, 5))
+    U0[..., 0], U0[..., 1] = 1.25, 0.8
+    U0[..., 2] = 0.3
+    s = gpu.Solver(30, 20, t_end=1.0, max_steps=4)
+    s.upload(U0)
+    s.begin()
+    s.advance(2)
+    assert s.sync()[0] == 2
+    assert np.all(s.download_frame()[..., 0] == 1.25)
+    g = s.download_frame_gray()
+    assert g.shape == (20, 30) and g.dtype == np.uint8 and not g.any()
     s.close()
 
 
