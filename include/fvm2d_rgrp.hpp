@@ -17,6 +17,7 @@
 #pragma once
 
 #include <algorithm>
+#include <charconv>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -134,15 +135,33 @@ class RGRP_2D_CUDA {
     }
     double getComputationTime_ms() { return ms_; }
     int getTimeStagesCount() { return steps_; }
+    // src/wrappers.cuh:370-397: rho_/p_/u_/v_<std::to_string(t_end)>s.txt in
+    // dir, one line per row, each value as the default ostream writes it
+    // (printf "%.6g", which std::to_chars in general format with precision
+    // 6 reproduces) and a space.  The four files are formatted on four
+    // threads.  Like the reference's, a file that cannot be opened is
+    // skipped silently.
     void writeFinalResultTo_txt(const std::string &dir) {
         const char *names[4] = {"rho", "p", "u", "v"};
-        for (int k = 0; k < 4; k++) {
-            std::ofstream f(dir + "/" + names[k] + "_" + std::to_string(endTime_) + "s.txt");
-            for (const auto &row : U_) {
-                for (const auto &e : row) f << e(k) << " ";
-                f << "\n";
-            }
-        }
+        std::future<void> files[4];
+        for (int k = 0; k < 4; k++)
+            files[k] = std::async(std::launch::async, [this, k, path = dir + "/" + names[k] + "_" +
+                                                                    std::to_string(endTime_) + "s.txt"] {
+                std::string text;
+                text.reserve(U_.size() * (U_.empty() ? 0 : U_[0].size()) * 10);
+                char num[32];
+                for (const auto &row : U_) {
+                    for (const auto &e : row) {
+                        const auto r = std::to_chars(num, num + sizeof(num), e(k), std::chars_format::general, 6);
+                        text.append(num, r.ptr);
+                        text.push_back(' ');
+                    }
+                    text.push_back('\n');
+                }
+                std::ofstream f(path, std::ios::binary);
+                f.write(text.data(), (std::streamsize)text.size());
+            });
+        for (auto &f : files) f.get();
     }
     void clear() {
         U0_.clear();

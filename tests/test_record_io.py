@@ -137,3 +137,36 @@ def test_oracle_record_frames_are_the_prefill_states(orc, problems, strategy):
         assert np.array_equal(frame[np.ix_(g, c)][..., :4], prev[np.ix_(g, c)][..., :4])
         # the fill changes the ghost cells: the frame is not the filled state
         assert not np.array_equal(frame[..., :4], after[..., :4])
+
+
+def test_cpp_header_text_files_match_ostream_format(tmp_path):
+    """The C++ header's writeFinalResultTo_txt (values formatted by
+    std::to_chars, four files on four threads) writes the bytes of the
+    reference's ostream format (src/wrappers.cuh:370-397: default precision,
+    "%.6g", a space after each value), which io.write_matrix_txt restates:
+    on seeded values over many magnitudes and the printf edge cases."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "txt_writer")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "cpp", "txt_writer.cpp"), "-pthread", "-ldl", "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rng = np.random.default_rng(4)
+    ny, nx = 37, 53
+    U = rng.standard_normal((ny, nx, 5)) * 10.0 ** rng.integers(-12, 12, size=(ny, nx, 5))
+    special = [0.0, -0.0, 1.0, 0.5, 123456.5, 1234565.0, 999999.5, 9999995.0, 1e-5, 1e-4, 1e21,
+               5e-324, 2.2250738585072014e-308, 0.30000000000000004, 1.0 / 3.0, 1e16]
+    U.reshape(-1)[:len(special)] = special
+    U[..., 4] = 0.0
+    src = tmp_path / "U.bin"
+    U.astype("<f8").tofile(src)
+    got, exp = tmp_path / "got", tmp_path / "exp"
+    got.mkdir()
+    exp.mkdir()
+    r = subprocess.run([exe, str(src), str(ny), str(nx), "0.3", str(got)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stderr
+    io.write_matrix_txt(str(exp), U, 0.3)
+    names = sorted(os.listdir(got))
+    assert names == ["p_0.300000s.txt", "rho_0.300000s.txt", "u_0.300000s.txt", "v_0.300000s.txt"]
+    for nm in names:
+        assert (got / nm).read_bytes() == (exp / nm).read_bytes(), nm
