@@ -22,12 +22,12 @@ def write_matrix_txt(dir: str, U: np.ndarray, cTime: float) -> None:
         U = np.zeros((0, 0, 5))
     tag = "%f" % cTime
     names = ("rho", "p", "u", "v")
+    U = np.asarray(U, dtype=np.float64)
+    row_fmt = "%.6g " * U.shape[1] + "\n"  # one format per row: half the time of one per value
     for k, name in enumerate(names):
         path = os.path.join(dir, f"{name}_{tag}s.txt")
         with open(path, "w") as f:
-            for row in U:
-                f.write("".join("%.6g " % x for x in row[:, k]))
-                f.write("\n")
+            f.write("".join(row_fmt % tuple(row) for row in U[:, :, k].tolist()))
 
 
 def normalize_u8(a: np.ndarray) -> np.ndarray:
