@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fvm2d.h"
@@ -414,13 +415,18 @@ struct Engine {
         if (i < 0 || i >= n || !U5) return fail(FVM2D_ERR_ARG, "bad instance or null state");
         const InstDesc &d = hd[i];
         HIPCHK(hipStreamSynchronize(stream));
-        for (int k = 0; k < 4; k++)
-            for (int j = 0; j < d.ny; j++) {
-                double *dst = stage + k * d.plane + (long long)j * d.pitch;
-                const double *s = U5 + ((long long)j * d.nx) * 5 + k;
-                for (int c = 0; c < d.nx; c++) dst[c] = s[5 * c];
-                for (int c = d.nx; c < d.pitch; c++) dst[c] = 0.0;
+        // one pass over the vec5d rows, four plane rows written at once
+        host_rows(d.ny, [&](int j0, int j1) {
+            for (int j = j0; j < j1; j++) {
+                const double *s = U5 + (long long)j * d.nx * 5;
+                double *dst[4];
+                for (int k = 0; k < 4; k++) dst[k] = stage + k * d.plane + (long long)j * d.pitch;
+                for (int c = 0; c < d.nx; c++)
+                    for (int k = 0; k < 4; k++) dst[k][c] = s[5 * c + k];
+                for (int k = 0; k < 4; k++)
+                    for (int c = d.nx; c < d.pitch; c++) dst[k][c] = 0.0;
             }
+        });
         HIPCHK(hipMemcpyAsync(d.buf0, stage, 4 * d.plane * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         begun = false;
@@ -454,14 +460,45 @@ struct Engine {
         const InstDesc &d = hd[i];
         HIPCHK(hipMemcpyAsync(stage, src, 4 * d.plane * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        for (int j = 0; j < d.ny; j++) {
-            double *o = U5 + (long long)j * d.nx * 5;
-            for (int c = 0; c < d.nx; c++) {
-                for (int k = 0; k < 4; k++) o[5 * c + k] = stage[k * d.plane + (long long)j * d.pitch + c];
-                o[5 * c + 4] = 0.0;
+        host_rows(d.ny, [&](int j0, int j1) {
+            for (int j = j0; j < j1; j++) {
+                double *o = U5 + (long long)j * d.nx * 5;
+                for (int c = 0; c < d.nx; c++) {
+                    for (int k = 0; k < 4; k++) o[5 * c + k] = stage[k * d.plane + (long long)j * d.pitch + c];
+                    o[5 * c + 4] = 0.0;
+                }
+            }
+        });
+        return FVM2D_OK;
+    }
+
+    // fn(j0, j1) over row blocks of [0, ny) on up to 8 host threads (the
+    // vec5d <-> plane reordering of large meshes is bound by host memory
+    // bandwidth, which one thread does not reach); small meshes stay on the
+    // calling thread.
+    template <class F>
+    static void host_rows(int ny, F &&fn) {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = (int)std::min<long long>({8, (long long)hw, std::max(1, ny / 128)});
+        if (nt <= 1) {
+            fn(0, ny);
+            return;
+        }
+        auto block = [&fn, nt, ny](int t) {
+            fn((int)((long long)ny * t / nt), (int)((long long)ny * (t + 1) / nt));
+        };
+        std::vector<std::thread> pool;
+        int t = 1;
+        for (; t < nt; t++) {
+            try {
+                pool.emplace_back(block, t);
+            } catch (...) {  // no thread to be had: the rest runs here
+                break;
             }
         }
-        return FVM2D_OK;
+        for (int r = t; r < nt; r++) block(r);
+        block(0);
+        for (auto &th : pool) th.join();
     }
 
     // Record-mode frame of instance i inside a time loop: the state after
