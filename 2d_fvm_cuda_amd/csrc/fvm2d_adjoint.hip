@@ -562,14 +562,24 @@ __global__ void __launch_bounds__(256) adj_copy_kernel(const double *const *__re
     const double2 *__restrict__ a = reinterpret_cast<const double2 *>(src[i]);
     double2 *__restrict__ b = reinterpret_cast<double2 *>(dst[i]);
     const long long n2 = count[i] / 2;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n2;
-         k += (long long)gridDim.x * blockDim.x)
-        b[k] = a[k];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // four 16-byte loads in flight per lane before their stores
+    for (; k + 3 * stride < n2; k += 4 * stride) {
+        const double2 x0 = a[k], x1 = a[k + stride], x2 = a[k + 2 * stride], x3 = a[k + 3 * stride];
+        b[k] = x0;
+        b[k + stride] = x1;
+        b[k + 2 * stride] = x2;
+        b[k + 3 * stride] = x3;
+    }
+    for (; k < n2; k += stride) b[k] = a[k];
 }
 
 void adj_launch_copy(const double *const *src, double *const *dst, const long long *count, int n,
                      long long max_count, hipStream_t s) {
-    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(64, (max_count / 2 + 255) / 256));
+    // about 4096 blocks over all meshes, each lane moving at least four 16-byte pieces
+    const long long per_mesh = std::max(16, 4096 / std::max(1, n));
+    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>(per_mesh, (max_count / 2 + 1023) / 1024));
     hipLaunchKernelGGL(adj_copy_kernel, dim3(bx, n), dim3(256), 0, s, src, dst, count);
 }
 void adj_launch_sweep(AdjDesc *descs, const int4 *tasks, int ntasks, int mode, int part_base,
