@@ -80,6 +80,23 @@ def test_checkpoint_budget_does_not_change_gradient(gpu, problems, adjmod):
     assert res[3][1]["recompute_steps"] == 0
 
 
+@pytest.mark.parametrize("steps,slots", [(1, 1), (1, 4), (2, 1), (2, 0), (3, 9)])
+def test_shortest_schedules_match_host_reference(gpu, orc, problems, adjmod, steps, slots):
+    """One and two steps, and slot budgets above the step count: the
+    schedule's edge cases give the host reference gradient."""
+    C0 = smooth_state(problems, 40, 34, seed=9)
+    gN = np.random.default_rng(steps).normal(size=C0.shape)
+    adj = adjmod.Adjoint([(34, 40)], [{}], nsteps=steps, checkpoint_slots=slots)
+    UN = adj.forward([C0])[0]
+    g0 = adj.backward([gN])[0]
+    st = adj.stats()
+    adj.close()
+    assert st["reverse_steps"] == steps
+    np.testing.assert_allclose(UN, orc.adj_forward(C0, steps), rtol=1e-12, atol=1e-13)
+    ref = orc.adj_backward(C0, steps, gN)
+    assert np.abs(g0 - ref).max() / np.abs(ref).max() <= 1e-11
+
+
 def _fd_pairs(adj, C0, target, g, npairs, seed, h=1e-6):
     """Central and one-sided differences of L = 1/2 sum (rho_N - target)^2
     at npairs random (component, row, column) pairs, each difference taken
