@@ -3,7 +3,8 @@ oracle (every entry point, every preset, sizes that are not multiples of
 16, the host reference adjoint), the engine's launch-layout logic and
 parameter checks, and the drop-in header's PNG frame writer, each built
 with g++ -fsanitize=address,undefined from a driver in tests/cpp/ and run;
-any report aborts the driver."""
+any report aborts the driver.  The header's threaded text writer also runs
+under ThreadSanitizer."""
 import os
 import subprocess
 
@@ -55,3 +56,25 @@ def test_frame_writer_under_sanitizers(tmp_path, stored):
     r = subprocess.run([exe, out, "513", "301"], capture_output=True, text=True, env=env, timeout=120)
     assert r.returncode == 0, r.stderr[-4000:]
     assert os.path.getsize(out) > 0
+
+
+def test_text_writer_under_thread_sanitizer(tmp_path):
+    """The drop-in header's writeFinalResultTo_txt formats its four files on
+    four threads: built with g++ -fsanitize=thread and run on a small state,
+    no data race is reported."""
+    import numpy as np
+    exe = str(tmp_path / "txt_tsan")
+    r = subprocess.run(["g++", "-std=c++17", "-fsanitize=thread", "-g", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "txt_writer.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    U = np.random.default_rng(2).standard_normal((30, 41, 5))
+    src = tmp_path / "U.bin"
+    U.astype("<f8").tofile(src)
+    out = tmp_path / "out"
+    out.mkdir()
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1:exitcode=66")
+    r = subprocess.run([exe, str(src), "30", "41", "0.1", str(out)], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stderr[-4000:]
+    assert len(os.listdir(out)) == 4
