@@ -555,7 +555,9 @@ struct AdjEngine {
         std::vector<AdjCtl> ac(n);
         HIPCHK(hipMemcpy(ac.data(), d_actl, n * sizeof(AdjCtl), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; i++)
-            if (ac[i].status & 1) return fail(FVM2D_ERR_NONFINITE, "non-finite wave speed in the reverse sweep");
+            if (ac[i].status & 1)
+                return fail(FVM2D_ERR_NONFINITE,
+                            "non-finite wave speed in the reverse sweep, or no cell attaining s_max");
         return FVM2D_OK;
     }
 };

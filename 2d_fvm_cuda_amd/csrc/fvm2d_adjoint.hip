@@ -435,11 +435,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // lam[argmax] += (sum of dt sensitivities) * d dt / d U[argmax], where
 // dt = CFL h / s_max and s_max = speed(U[argmax]).  One block per instance;
 // the sum runs in a fixed order (four strided partial sums per thread, a
-// fixed tree over the block), so the result is reproducible.
+// butterfly over each wave, then one wave over the wave sums), so the
+// result is reproducible.  The arg-max cell's state and output adjoint are
+// fetched by the first lane before the partials are summed.  A step whose
+// arg-max was not found (no cell's speed equals the forward loop's s_max)
+// adds nothing and sets status bit 1.
 __global__ void __launch_bounds__(1024) adj_dt_term_kernel(AdjDesc *__restrict__ descs) {
-    __shared__ double red[1024];
+    __shared__ double red[1024 / kWave];
     AdjDesc &d = descs[blockIdx.x];
     const int T = blockDim.x;
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const long long P = d.plane;
+    long long o = -1;
+    Q g{0.0, 0.0, 0.0, 0.0}, lam{0.0, 0.0, 0.0, 0.0};
+    if (threadIdx.x == 0) {
+        const long long k = d.ctl->argmax;
+        if (k >= 0 && k < (long long)d.nx * d.ny) {
+            o = (k / d.nx) * d.pitch + k % d.nx;
+            g = speed_grad(d.U[o], d.U[P + o], d.U[2 * P + o], d.U[3 * P + o], d.gas);
+            lam = Q{d.lam_out[o], d.lam_out[P + o], d.lam_out[2 * P + o], d.lam_out[3 * P + o]};
+        }
+    }
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (int r = 0; r < 3; r++) {
         const int lo = d.part_lo[r], hi = d.part_hi[r];
@@ -454,26 +470,28 @@ __global__ void __launch_bounds__(1024) adj_dt_term_kernel(AdjDesc *__restrict__
         if (p + T < hi) a1 += d.dtpart[p + T];
         if (p + 2 * T < hi) a2 += d.dtpart[p + 2 * T];
     }
-    red[threadIdx.x] = (a0 + a1) + (a2 + a3);
+    double a = (a0 + a1) + (a2 + a3);
+    for (int w = kWave / 2; w > 0; w >>= 1) a += __shfl_xor(a, w, kWave);  // every lane: the wave's sum
+    if (lane == 0) red[wid] = a;
     __syncthreads();
-    for (int w = T / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double dtb = red[0];
-        const long long k = d.ctl->argmax;
-        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
-        const long long o = (long long)j * d.pitch + i;
-        const long long P = d.plane;
-        const Q g = speed_grad(d.U[o], d.U[P + o], d.U[2 * P + o], d.U[3 * P + o], d.gas);
-        const double f = dtb * (-d.fctl->dt / d.fctl->smax);
-        d.lam_out[o] += f * g.r;
-        d.lam_out[P + o] += f * g.p;
-        d.lam_out[2 * P + o] += f * g.un;
-        d.lam_out[3 * P + o] += f * g.ut;
-        d.ctl->dtbar = dtb;
-        d.ctl->argmax = kNoCell;  // for the next reverse step's search
+    if (wid == 0) {
+        const int nw = T / kWave;
+        double b = lane < nw ? red[lane] : 0.0;
+        for (int w = kWave / 2; w > 0; w >>= 1) b += __shfl_xor(b, w, kWave);
+        if (lane == 0) {
+            const double dtb = b;
+            if (o >= 0) {
+                const double f = dtb * (-d.fctl->dt / d.fctl->smax);
+                d.lam_out[o] = lam.r + f * g.r;
+                d.lam_out[P + o] = lam.p + f * g.p;
+                d.lam_out[2 * P + o] = lam.un + f * g.un;
+                d.lam_out[3 * P + o] = lam.ut + f * g.ut;
+            } else {
+                atomicOr(&d.ctl->status, 1);
+            }
+            d.ctl->dtbar = dtb;
+            d.ctl->argmax = kNoCell;  // for the next reverse step's search
+        }
     }
 }
 

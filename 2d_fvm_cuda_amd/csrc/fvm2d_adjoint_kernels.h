@@ -12,7 +12,7 @@ struct AdjCtl {
     double dtbar;      // total sensitivity of the loss to dt_n
     long long argmax;  // linear index (j * nx + i) of the first cell attaining s_max
                        // (kNoCell between reverse steps)
-    int status;        // bit 0: non-finite wave speed
+    int status;        // bit 0: non-finite wave speed, or no cell attaining s_max
     int pad;
 };
 constexpr long long kNoCell = 0x7fffffffffffffffll;
