@@ -61,6 +61,7 @@ struct AdjEngine {
     bool launch_chosen = false;  // the forward engine's launch geometry is fixed
 
     ~AdjEngine() {
+        if (fwd.stream) hipStreamSynchronize(fwd.stream);  // reverse steps may still be queued
         for (void *p : allocs) hipFree(p);
         if (r0) hipEventDestroy(r0);
         if (r1) hipEventDestroy(r1);

@@ -97,6 +97,9 @@ struct Engine {
     ~Engine() { release(); }
 
     void release() {
+        // work still queued on the stream (launches enqueued without a sync)
+        // may use every buffer and the graph below
+        if (stream) hipStreamSynchronize(stream);
         if (gexec) hipGraphExecDestroy(gexec);
         gexec = nullptr;
         if (pool) hipFree(pool);
@@ -400,7 +403,10 @@ struct Engine {
     }
 
     void drop_graph() {
-        if (gexec) hipGraphExecDestroy(gexec);
+        if (gexec) {
+            hipStreamSynchronize(stream);  // no replay of it may still be queued
+            hipGraphExecDestroy(gexec);
+        }
         gexec = nullptr;
         gsteps = 0;
     }
