@@ -75,7 +75,6 @@ struct Engine {
     Ctl *d_ctl = nullptr;
     Ctl *h_ctl = nullptr;            // pinned read-back
     Sched *d_sched = nullptr;
-    CtlSlot *d_slots = nullptr;      // n * kCtlSlots end-of-task arrival slots
     double *d_t0 = nullptr, *h_t0 = nullptr;  // per-instance start times
     int4 *d_tasks = nullptr;
     std::vector<int4> h_tasks;       // host copy of d_tasks (source of its async upload)
@@ -106,8 +105,6 @@ struct Engine {
         if (d_desc) hipFree(d_desc);
         if (d_ctl) hipFree(d_ctl);
         if (d_sched) hipFree(d_sched);
-        if (d_slots) hipFree(d_slots);
-        d_slots = nullptr;
         if (d_tasks) hipFree(d_tasks);
         if (h_ctl) hipHostFree(h_ctl);
         if (d_t0) hipFree(d_t0);
@@ -195,9 +192,6 @@ struct Engine {
         HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
         HIPCHK(hipMalloc(&d_sched, sizeof(Sched)));
         HIPCHK(hipMemsetAsync(d_sched, 0, sizeof(Sched), stream));
-        HIPCHK(hipMalloc(&d_slots, (size_t)n * kCtlSlots * sizeof(CtlSlot)));
-        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
-        for (int i = 0; i < n; i++) hd[i].slots = d_slots + (size_t)i * kCtlSlots;
         HIPCHK(hipHostMalloc(&h_ctl, n * sizeof(Ctl), hipHostMallocDefault));
         HIPCHK(hipMalloc(&d_t0, n * sizeof(double)));
         HIPCHK(hipHostMalloc(&h_t0, n * sizeof(double), hipHostMallocDefault));
@@ -601,7 +595,6 @@ struct Engine {
     // 0): enqueues the reset without waiting for the host.
     int begin_device(const double *d_times) {
         HIPCHK(hipMemsetAsync(d_ctl, 0, n * sizeof(Ctl), stream));
-        HIPCHK(hipMemsetAsync(d_slots, 0, (size_t)n * kCtlSlots * sizeof(CtlSlot), stream));
         for (int i = 0; i < n; i++) launch_speed(exact, d_desc, d_ctl, i, 0, hd[i].ny, stream);
         launch_ctl_start(d_desc, d_ctl, n, d_times, stream);
         HIPCHK(hipGetLastError());
@@ -610,8 +603,7 @@ struct Engine {
     }
 
     // Start time loops from control blocks saved after an earlier loop
-    // (device array of n), each instance's state already in buf0.  The
-    // arrival slots are zero: every completed launch clears them.
+    // (device array of n), each instance's state already in buf0.
     int begin_from_ctl(const Ctl *d_saved) {
         launch_ctl_restore(d_ctl, d_saved, n, stream);
         HIPCHK(hipGetLastError());
@@ -620,7 +612,7 @@ struct Engine {
     }
 
     void launch_one() {
-        launch_step(exact, d_desc, d_ctl, d_tasks, ntasks, d_sched, persistent, grid, wpb, stream);
+        launch_step(exact, d_desc, d_ctl, n, d_tasks, ntasks, d_sched, persistent, grid, wpb, stream);
     }
 
     int ensure_graph(int steps) {

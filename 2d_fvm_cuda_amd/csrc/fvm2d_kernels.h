@@ -7,8 +7,10 @@
 
 namespace fvm2d {
 
-// A task is one wave's strip: {instance, column window, first row, end row}.
-void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
+// One time step of ninst instances: the step kernel (a task is one wave's
+// strip: {instance, column window, first row, end row}), then the advance
+// kernel that moves each instance's control block to the next step.
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, int ninst, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream);
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
                   long long nrows, hipStream_t stream);

@@ -2,8 +2,8 @@
 // memory allocation, for callers that cannot allocate (code running under
 // stream capture setup, tight serving loops, harnesses that forbid
 // hipMalloc).  Everything the time loop needs beyond the caller's state --
-// the second time-loop buffer, the mesh descriptor, its control block,
-// arrival slots and task tickets, and the task list -- lives in one
+// the second time-loop buffer, the mesh descriptor, its control block, the
+// task ticket and the task list -- lives in one
 // file-scope device array.  Each GPU holds its own copy of that array for
 // the life of the process; host code reaches it only through the address
 // hipGetSymbolAddress returns for the current device.
@@ -22,12 +22,11 @@ constexpr size_t kAlign = 256;
 
 constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// Layout of the array: [state buffer][InstDesc][Ctl][kCtlSlots CtlSlot][Sched][tasks]
+// Layout of the array: [state buffer][InstDesc][Ctl][Sched][tasks]
 constexpr size_t kOffState = 0;
 constexpr size_t kOffDesc = align_up(kOffState + 4 * (size_t)kNoAllocMaxCells * sizeof(double));
 constexpr size_t kOffCtl = align_up(kOffDesc + sizeof(InstDesc));
-constexpr size_t kOffSlots = align_up(kOffCtl + sizeof(Ctl));
-constexpr size_t kOffSched = align_up(kOffSlots + kCtlSlots * sizeof(CtlSlot));
+constexpr size_t kOffSched = align_up(kOffCtl + sizeof(Ctl));
 constexpr size_t kOffTasks = align_up(kOffSched + sizeof(Sched));
 constexpr size_t kScratchBytes = align_up(kOffTasks + kNoAllocMaxTasks * sizeof(int4));
 
@@ -52,19 +51,17 @@ static int scratch_ptr(unsigned char **out) {
     return FVM2D_OK;
 }
 
-// Writes the descriptor, clears the control block, arrival slots and
-// tickets (a kernel, not a memset), and lays out the strips: task k is
+// Writes the descriptor, clears the control block and the task ticket (a
+// kernel, not a memset), and lays out the strips: task k is
 // window k mod nw, rows [2 + s R, min(ny - 2, 2 + (s + 1) R)) with s = k / nw.
 __global__ void __launch_bounds__(256) noalloc_init_kernel(InstDesc d, InstDesc *desc, Ctl *ctl,
-                                                           CtlSlot *slots, Sched *sched, int4 *tasks,
-                                                           int ntasks) {
+                                                           Sched *sched, int4 *tasks, int ntasks) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         *desc = d;
         *ctl = Ctl{};
         *sched = Sched{0u, 0u};
     }
-    if (k < kCtlSlots) slots[k] = CtlSlot{};
     if (k < ntasks) {
         const int w = k % d.nw, s = k / d.nw;
         const int r0 = 2 + s * d.R;
@@ -93,7 +90,6 @@ extern "C" int fvm2d_solve_device_noalloc(double *d_planes, int nx, int ny, cons
     d.plane = (long long)nx * ny;
     d.buf0 = d_planes;
     d.buf1 = reinterpret_cast<double *>(base + kOffState);
-    d.slots = reinterpret_cast<CtlSlot *>(base + kOffSlots);
     // strips of R rows over the nw windows, R the lowest height (>= 8)
     // whose task count fits the task array
     const int H = ny - 4;
@@ -111,8 +107,8 @@ extern "C" int fvm2d_solve_device_noalloc(double *d_planes, int nx, int ny, cons
     int4 *d_tasks = reinterpret_cast<int4 *>(base + kOffTasks);
     const bool exact = p->mode == FVM2D_MODE_EXACT;
     hipStream_t st = nullptr;
-    hipLaunchKernelGGL(noalloc_init_kernel, dim3((std::max(ntasks, kCtlSlots) + 255) / 256), dim3(256), 0, st,
-                       d, d_desc, d_ctl, d.slots, d_sched, d_tasks, ntasks);
+    hipLaunchKernelGGL(noalloc_init_kernel, dim3((ntasks + 255) / 256), dim3(256), 0, st, d, d_desc, d_ctl,
+                       d_sched, d_tasks, ntasks);
     launch_speed(exact, d_desc, d_ctl, 0, 0, ny, st);
     launch_ctl_start(d_desc, d_ctl, 1, nullptr, st);
     HIPCHK(hipGetLastError());
@@ -129,7 +125,7 @@ extern "C" int fvm2d_solve_device_noalloc(double *d_planes, int nx, int ny, cons
             if (r < 1e9) rem = std::min(rem, (long long)(r * 1.02) + 2);
         }
         const int chunk = (int)std::max(1ll, std::min(rem, 4096ll));
-        for (int k = 0; k < chunk; k++) launch_step(exact, d_desc, d_ctl, d_tasks, ntasks, d_sched, 0, grid, 4, st);
+        for (int k = 0; k < chunk; k++) launch_step(exact, d_desc, d_ctl, 1, d_tasks, ntasks, d_sched, 0, grid, 4, st);
         HIPCHK(hipGetLastError());
     }
     // the final state is buf[steps & 1]

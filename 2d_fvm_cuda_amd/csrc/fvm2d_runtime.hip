@@ -65,11 +65,9 @@ int fvm2d_set_params(fvm2d_solver *s, const fvm2d_params *p) {
     }
     s->eng.prm[0] = *p;
     double *b0 = s->eng.hd[0].buf0, *b1 = s->eng.hd[0].buf1;
-    CtlSlot *sl = s->eng.hd[0].slots;
     Engine::fill_desc(s->eng.hd[0], s->nx, s->ny, *p);
     s->eng.hd[0].buf0 = b0;
     s->eng.hd[0].buf1 = b1;
-    s->eng.hd[0].slots = sl;
     return s->eng.set_launch(s->eng.R, s->eng.wpb, s->eng.persistent);
 }
 

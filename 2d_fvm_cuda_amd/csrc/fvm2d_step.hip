@@ -156,7 +156,6 @@ __device__ __forceinline__ void advance_ctl(const InstDesc &d, Ctl *c, double t_
     c->done = done;
     c->status |= status;
     c->smax_next = 0ull;
-    c->waves_done = 0u;
 }
 
 // A control-block field read through the scalar cache.  The fields a task
@@ -170,10 +169,10 @@ __device__ __forceinline__ T ctl_field(const T *p) {
 
 // ---------------------------------------------------------------- the step
 // One task: window w (columns w * kOwn .. + kOwn - 1 owned), output rows
-// [r0, r1); local is the task's index among its instance's tasks.
+// [r0, r1).
 template <bool Exact>
-__device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, int local,
-                          int lane, const Xch &xa, const Xch &xb) {
+__device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, int lane, const Xch &xa,
+                          const Xch &xb) {
     const int nx = d.nx, ny = d.ny;
     const long long P = d.plane;
     const double t = ctl_field(&ctl->t), dt = ctl_field(&ctl->dt);
@@ -253,33 +252,15 @@ __device__ void step_task(const InstDesc &d, Ctl *ctl, int w, int r0, int r1, in
         }
     }
 
+    // The wave's speed maximum goes into the instance's accumulator with one
+    // device-scope atomic whose result is not waited for; the advance kernel
+    // launched after this one reads the total.  (An earlier build let the
+    // last task of the launch advance the control block: its arrival
+    // counters needed a device-scope fence per wave, which on this
+    // many-XCD part writes back and invalidates the XCD's L2 under the
+    // other waves' feet: 2000 x 2000 ran 181 instead of 150 us per step.)
     mx = wave_max_u64(mx);
-    if (lane == 0) {
-        // two-level arrival: task -> its slot -> the instance
-        const int nt = d.ntask;
-        const int slot = local % kCtlSlots;
-        const int nslots = nt < kCtlSlots ? nt : kCtlSlots;
-        const int per = nt / kCtlSlots + (slot < nt % kCtlSlots ? 1 : 0);
-        CtlSlot *sl = d.slots + slot;
-        atomicMax(&sl->smax, mx);
-        __threadfence();
-        const unsigned int prev = atomicAdd(&sl->count, 1u);
-        if (prev == (unsigned int)(per - 1)) {
-            // last task of the slot: take the slot's maximum and clear the
-            // slot for the next launch
-            __threadfence();
-            const unsigned long long sm = atomicExch(&sl->smax, 0ull);
-            atomicExch(&sl->count, 0u);
-            atomicMax(&ctl->smax_next, sm);
-            __threadfence();
-            const unsigned int prev2 = atomicAdd(&ctl->waves_done, 1u);
-            if (prev2 == (unsigned int)(nslots - 1)) {
-                __threadfence();
-                const unsigned long long sb = atomicOr(&ctl->smax_next, 0ull);
-                advance_ctl(d, ctl, t_new, steps + 1, sb);
-            }
-        }
-    }
+    if (lane == 0) atomicMax(&ctl->smax_next, mx);
 }
 
 template <bool Exact>
@@ -309,21 +290,26 @@ __global__ void __launch_bounds__(256) fvm_step_kernel(const InstDesc *__restric
         const int r1 = __builtin_amdgcn_readfirstlane(task.w);
         Ctl *c = ctls + inst;
         const InstDesc &d = descs[inst];
-        // the tasks of an instance hold positions task_base .. task_base + ntask - 1
-        if (!ctl_field(&c->done)) step_task<Exact>(d, c, w, r0, r1, tk - d.task_base, lane, xa, xb);
+        if (!ctl_field(&c->done)) step_task<Exact>(d, c, w, r0, r1, lane, xa, xb);
         if (!persistent) break;
         unsigned int v = 0;
         if (lane == 0) v = atomicAdd(&sched->ticket, 1u);
         tk = __builtin_amdgcn_readfirstlane((int)__shfl(v, 0, kWave));
     }
-    if (persistent && lane == 0) {
-        const unsigned int total = gridDim.x * wpb;
-        const unsigned int e = atomicAdd(&sched->exited, 1u);
-        if (e == total - 1) {  // every wave has left the task loop
-            atomicExch(&sched->ticket, 0u);
-            atomicExch(&sched->exited, 0u);
-        }
-    }
+}
+
+// After every step launch: each instance still running takes its step --
+// t += dt, the step count, the dt rule on the new state's s_max (the
+// accumulator the step's waves filled) -- and the persistent launch's task
+// ticket is reset.  One thread per instance.
+__global__ void fvm_advance_kernel(const InstDesc *__restrict__ descs, Ctl *__restrict__ ctls, int ninst,
+                                   Sched *sched) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && sched) sched->ticket = 0u;
+    if (i >= ninst) return;
+    Ctl *c = ctls + i;
+    if (c->done) return;
+    advance_ctl(descs[i], c, c->t + c->dt, c->steps + 1, c->smax_next);
 }
 
 // ---------------------------------------------------------------- set-up
@@ -379,7 +365,6 @@ __global__ void fvm_ctl_restore_kernel(Ctl *__restrict__ ctls, const Ctl *__rest
     Ctl c = saved[i];
     c.steps = 0;
     c.smax_next = 0ull;
-    c.waves_done = 0u;
     ctls[i] = c;
 }
 
@@ -530,7 +515,7 @@ void launch_frame_gray(const InstDesc &d, const double *frame, unsigned long lon
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks, int ntasks,
+void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, int ninst, const int4 *tasks, int ntasks,
                  Sched *sched, int persistent, int grid, int wpb, hipStream_t stream) {
     const dim3 block(wpb * kWave);
     if (exact)
@@ -539,6 +524,8 @@ void launch_step(bool exact, const InstDesc *descs, Ctl *ctls, const int4 *tasks
     else
         hipLaunchKernelGGL(fvm_step_kernel<false>, dim3(grid), block, 0, stream, descs, ctls, tasks,
                            ntasks, sched, persistent);
+    hipLaunchKernelGGL(fvm_advance_kernel, dim3((ninst + 255) / 256), dim3(256), 0, stream, descs, ctls,
+                       ninst, persistent ? sched : nullptr);
 }
 
 void launch_speed(bool exact, const InstDesc *descs, Ctl *ctls, int inst, int use_buf1,
