@@ -214,10 +214,15 @@ struct Engine {
     // one or two rounds are filled exactly.  At 2000 x 2000 on 256 CUs this
     // is one round at R = 34, the geometry the autotuner picks.  A strip
     // also pays a fixed cost (its halo rows and the march's start), so when
-    // one round would give strips below kMinOneMeshRows rows, half, a
-    // quarter, ... of a round of taller strips is used instead (measured
-    // per step: 1000^2 0.101 -> 0.078 ms at R = 18, 768^2 0.089 -> 0.058 ms
-    // at R = 10, 512^2 0.061 -> 0.045 ms at R = 10).
+    // one round would give strips below kMinOneMeshRows rows, half a round
+    // (one wave per SIMD) of twice as high strips is used, at least 4 rows
+    // (measured per step, against one round and against the sixteenth of a
+    // round of 11-row strips an earlier rule went down to: 256^2 21.5 us at
+    // R = 4 against 39.9 us at R = 11, 512^2 30.6 us at R = 5 against 38.1
+    // us, 640^2 35.3 us at R = 7 against 41.7 us for one round, 768^2 42.3
+    // us at R = 10 against 48.8 us, 1000^2 59.1 us at R = 17 against 59.3
+    // us; from 1100^2 on one round is faster, 1200^2 66.8 us at R = 12
+    // against 78.1 us at R = 24).
     static constexpr long long kMinOneMeshRows = 10;
     int one_mesh_rows() const {
         const InstDesc &d = hd[0];
@@ -230,11 +235,8 @@ struct Engine {
             const long long r = rows_for(per_window);
             if (r > 96) continue;
             if (k == 1 && r < kMinOneMeshRows) {
-                for (long long div = 2; div <= 64; div *= 2) {
-                    const long long pw = std::max(1ll, slots / div / d.nw);
-                    const long long rr = rows_for(pw);
-                    if (rr >= kMinOneMeshRows || pw == 1) return (int)std::min(96ll, rr);
-                }
+                const long long pw = std::max(1ll, slots / 2 / d.nw);
+                return (int)std::min(96ll, std::max(4ll, rows_for(pw)));
             }
             return (int)std::max<long long>(4, r);
         }
@@ -752,6 +754,11 @@ struct Engine {
             }
             for (int k = 1; k <= 8 && n == 1; k++) {
                 const long long strips = std::max<long long>(1, k * slots / std::max<long long>(1, cols));
+                const int r = (int)((rows_max + strips - 1) / strips);
+                if (r >= 4 && r <= 96) Rs.push_back(r);
+            }
+            if (n == 1) {  // half a round: one wave per SIMD
+                const long long strips = std::max<long long>(1, slots / 2 / std::max<long long>(1, cols));
                 const int r = (int)((rows_max + strips - 1) / strips);
                 if (r >= 4 && r <= 96) Rs.push_back(r);
             }
